@@ -1,0 +1,187 @@
+"""Headline benchmark: particle-steps/s of the PM gravity step on one MI355X.
+
+Workload (default): 2048^2 lattice, 100,000,000 particles, CIC, two colliding
+exponential disks (r_d = 96 cells, 5e7 unit-mass particles each, centres
+(700,1024) and (1348,1024), +/-0.3 v_c(r_d) approach), dt = 0.002.  One step =
+deposit + FFT field solve + force gather / kick-drift over the full particle
+set, inputs resident in HBM.  W warm-up steps, then K timed steps bracketed by
+device synchronisation.  Per-kernel durations come from HIP events recorded on
+the simulation's own stream inside the timed region.
+
+Prints ONE JSON line (rank 0, N=1).  --gpus N>1 exits: this engine runs one
+simulation per GPU and has no multi-GPU path.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @100M CIC 2048²; ensemble steps/s + p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (lattice, particles, mode, dt, generator)
+    "colliding_disks_2048_100M_cic": (2048, 100_000_000, 1, 0.002, "colliding"),
+    "colliding_disks_2048_100M_ngp": (2048, 100_000_000, 0, 0.002, "colliding"),
+    "single_disk_2048_20M_cic": (2048, 20_000_000, 1, 0.005, "single"),
+}
+
+
+def algorithmic_bytes(kernel, n, np_):
+    """Algorithmic HBM bytes per launch: particle streams the kernel must move
+    plus the lattice it must read or write once (implementation extras such as
+    atomics' read-modify-write or re-reads are not counted)."""
+    nk = n // 2 + 1
+    return {
+        "clear": 8 * n * n,
+        "deposit": 12 * np_ + 8 * n * n,          # read x, y, m; write int64 lattice
+        "row_r2c": 8 * n * n + 8 * nk * n,        # read int64 lattice, write half spectrum
+        "col_conv": 8 * nk * n + 4 * nk * n + 16 * nk * n,   # read H, read T, write H and Hfy
+        "row_c2r": 16 * nk * n + 4 * n * n + 8 * n * n,      # read H, Hfy; write Phi and force
+        "edge": 8 * 8 * n,
+        "stencil": 4 * n * n + 8 * n * n,
+        "push": 32 * np_ + 8 * n * n,             # read/write x, y, vx, vy; read force lattice
+    }[kernel]
+
+
+def make_ics(gen, np_, seed):
+    from cosmossimulationwithcuda_amd import ics
+    if gen == "colliding":
+        return ics.colliding_disks(np_, seed)
+    return ics.single_disk(np_, seed)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(n, mode, dt, arrs, steps, threads):
+    """The CPU oracle step (int64 deposit, FP64 FFT field, FP32 push) on the
+    same particles, timed on this host's cores."""
+    from oracle import oracle as orc
+    orc.set_threads(threads)
+    x, y, vx, vy, m = arrs
+    s = orc.MixedStepper(n, mode, np.float32(dt), 1.0, 1.0, 1e-3, 0, 30, x, y, vx, vy, m)
+    s.step(1)                         # warm-up (page-in, thread start)
+    t0 = time.perf_counter()
+    s.step(steps)
+    el = time.perf_counter() - t0
+    return len(x) * steps / el, el, orc.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="colliding_disks_2048_100M_cic", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--seed", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs each simulation on one GPU; multi-GPU runs are not supported "
+              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    from cosmossimulationwithcuda_amd import pm
+
+    n, np_, mode, dt, gen = WORKLOADS[args.workload]
+    t_ic = time.perf_counter()
+    arrs = make_ics(gen, np_, args.seed)
+    t_ic = time.perf_counter() - t_ic
+    print(f"# ics {args.workload}: {np_} particles in {t_ic:.1f}s", file=sys.stderr, flush=True)
+
+    cfg = pm.default_config(n_lattice=n, n_particles=np_, mode=mode, dt=dt, gravity=1.0, green_origin=1.0,
+                            deconv_eps=1e-3)
+    sim = pm.Simulation(cfg)
+    sim.load(*arrs)
+    sim.step(args.warmup)
+    sim.sync()
+    print(f"# warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    sim.profile(True)
+    sim.profile_reset()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    prof = sim.profile_read()
+    sim.profile(False)
+    diag = sim.diagnostics()
+    print(f"# timed {args.steps} steps in {el:.3f}s", file=sys.stderr, flush=True)
+
+    kernels = {}
+    for name, (ms, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        avg_ms = ms / cnt
+        gb = algorithmic_bytes(name, n, np_) / 1e9
+        kernels[name] = {"avg_ms": round(avg_ms, 5), "launches": cnt, "algorithmic_GB": round(gb, 4),
+                         "GB_s": round(gb / (avg_ms / 1e3), 1)}
+    dom = max(("deposit", "push"), key=lambda k: kernels.get(k, {"avg_ms": 0})["avg_ms"])
+    kd = kernels[dom]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(args.workload, {}).get(dom)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": kd["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(kd["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom}
+
+    value = np_ * args.steps / el
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "lattice": n, "particles": np_,
+                   "mode": "cic" if mode == 1 else "ngp", "dt": dt, "force_mode": "spectral"},
+        "roofline": roofline,
+        "kernels": kernels,
+        "diagnostics": {k: diag[k] for k in ("mass", "kinetic", "potential", "nonfinite", "in_bounds")},
+    }
+    del sim
+
+    if args.cpu_steps > 0:
+        threads = args.cpu_threads or os.cpu_count()
+        print(f"# cpu baseline: {args.cpu_steps} oracle steps on {threads} threads", file=sys.stderr, flush=True)
+        v, el_cpu, used = cpu_baseline(n, mode, dt, arrs, args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": used, "kind": "port",
+                               "sample": f"{args.cpu_steps} full steps of the same workload ({np_} particles, "
+                                         f"{n}^2) after 1 warm-up step, {el_cpu:.1f}s; oracle/pm_oracle.c "
+                                         f"(int64 deposit, FP64 FFT field, FP32 push), host {cpu_model()}"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,541 @@
+// C-ABI layer of libpm.so (declared in include/pm.h): handle lifecycle,
+// uploads/read-back, step orchestration on one HIP stream, diagnostics and
+// per-kernel event timing.  Replaces the device side of the reference's
+// struct Universe (CosmosCuda.cuh:830-1428): ctor device setup 969-1035,
+// step methods 1209-1300, worker-loop body 1336-1341, dtor 1396-1427.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pm.h"
+#include "pm_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;   // errors of calls made without a valid handle
+
+struct ProfRec {
+    int kernel;
+    hipEvent_t start, end;
+};
+
+}  // namespace
+
+struct pm_sim {
+    pm_config cfg{};
+    hipStream_t stream = nullptr;
+    int64_t cap = 0, np = 0;
+    int shift = 40;
+    float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *m = nullptr;
+    int64_t* rho = nullptr;
+    float2 *H = nullptr, *Hfy = nullptr;
+    float* T = nullptr;
+    float* dk = nullptr;
+    float2* tw = nullptr;
+    float* phi = nullptr;
+    float2* force = nullptr;
+    double* diag = nullptr;
+    int64_t bytes = 0;
+    std::vector<void*> allocs;
+    std::string err;
+    bool profiling = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[PM_K_COUNT] = {};
+    int64_t prof_n[PM_K_COUNT] = {};
+};
+
+namespace {
+
+pm_status fail(pm_sim* h, pm_status st, const std::string& msg) {
+    if (h) h->err = msg;
+    g_last_error = msg;
+    return st;
+}
+
+pm_status hip_fail(pm_sim* h, hipError_t e, const char* what) {
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(h, e == hipErrorOutOfMemory ? PM_ENOMEM : PM_EDEVICE, msg);
+}
+
+#define PM_HIP(h, call)                                        \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return hip_fail((h), e_, #call); \
+    } while (0)
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+pm_status validate(const pm_config* c, std::string* why) {
+    if (!c) { *why = "null config"; return PM_EINVAL; }
+    if (!pmk::lattice_supported(c->n_lattice)) { *why = "n_lattice must be a power of two in [64, 8192]"; return PM_EINVAL; }
+    if (c->mode != PM_NGP && c->mode != PM_CIC) { *why = "mode must be PM_NGP or PM_CIC"; return PM_EINVAL; }
+    if (c->n_particles < 0 || c->n_particles > ((int64_t)1 << 34)) { *why = "n_particles out of range"; return PM_EINVAL; }
+    if (!std::isfinite(c->dt) || !std::isfinite(c->gravity)) { *why = "dt and gravity must be finite"; return PM_EINVAL; }
+    if (!(c->deconv_eps >= 0.0) || !std::isfinite(c->green_origin)) { *why = "bad deconv_eps or green_origin"; return PM_EINVAL; }
+    if (c->wrap_mode != PM_WRAP_KEEP_VELOCITY && c->wrap_mode != PM_WRAP_REFERENCE) { *why = "bad wrap_mode"; return PM_EINVAL; }
+    if (c->force_mode != PM_FORCE_SPECTRAL && c->force_mode != PM_FORCE_STENCIL) { *why = "bad force_mode"; return PM_EINVAL; }
+    if (c->precision != PM_FFT_FP32) { *why = "precision: only PM_FFT_FP32 is built"; return PM_EINVAL; }
+    if (c->mass_shift < 0 || c->mass_shift > 62) { *why = "mass_shift must be in [0, 62]"; return PM_EINVAL; }
+    if (c->wrap_mode == PM_WRAP_REFERENCE && c->dt == 0.0f) { *why = "wrap_mode reference needs dt != 0"; return PM_EINVAL; }
+    return PM_OK;
+}
+
+int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+struct Layout {
+    int64_t particles, rho, H, T, dk, tw, phi, force, diag, total, setup_work;
+};
+
+Layout layout_of(const pm_config* c) {
+    const int64_t n = c->n_lattice, nk = n / 2 + 1, cap = std::max<int64_t>(c->n_particles, 1);
+    Layout L{};
+    L.particles = align256(4 * cap);
+    L.rho = align256(8 * n * n);
+    L.H = align256(8 * nk * n);
+    L.T = align256(4 * nk * n);
+    L.dk = align256(4 * n);
+    L.tw = align256(8 * n);
+    L.phi = align256(4 * n * n);
+    L.force = align256(8 * n * n);
+    const int64_t dblocks = std::max(pmk::diag_blocks(cap), pmk::diag_lattice_blocks((int)n));
+    L.diag = align256(8 * 8 * dblocks);
+    L.total = 5 * L.particles + L.rho + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force + L.diag;
+    L.setup_work = 16 * n * n;
+    return L;
+}
+
+pm_status dalloc(pm_sim* h, void** p, int64_t bytes) {
+    hipError_t e = hipMalloc(p, (size_t)bytes);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMalloc");
+    h->allocs.push_back(*p);
+    h->bytes += bytes;
+    return PM_OK;
+}
+
+void release(pm_sim* h) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& r : h->recs) { (void)hipEventDestroy(r.start); (void)hipEventDestroy(r.end); }
+    for (auto e : h->pool) (void)hipEventDestroy(e);
+    for (void* p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    h->stream = nullptr;
+}
+
+// ---- per-kernel event timing ------------------------------------------------
+hipEvent_t pool_event(pm_sim* h) {
+    if (!h->pool.empty()) { hipEvent_t e = h->pool.back(); h->pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct KernelTimer {
+    pm_sim* h; int kernel; hipEvent_t s = nullptr;
+    KernelTimer(pm_sim* hh, int k) : h(hh), kernel(k) {
+        if (h->profiling && (s = pool_event(h))) (void)hipEventRecord(s, h->stream);
+    }
+    ~KernelTimer() {
+        if (!s) return;
+        hipEvent_t e = pool_event(h);
+        if (!e) { h->pool.push_back(s); return; }
+        (void)hipEventRecord(e, h->stream);
+        h->recs.push_back({kernel, s, e});
+    }
+};
+
+pm_status resolve_profile(pm_sim* h) {
+    if (h->recs.empty()) return PM_OK;
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    for (auto& r : h->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.start, r.end) == hipSuccess) {
+            h->prof_ms[r.kernel] += ms;
+            h->prof_n[r.kernel] += 1;
+        }
+        h->pool.push_back(r.start);
+        h->pool.push_back(r.end);
+    }
+    h->recs.clear();
+    return PM_OK;
+}
+
+// ---- stages -----------------------------------------------------------------
+pm_status stage_deposit(pm_sim* h) {
+    const int n = h->cfg.n_lattice;
+    {
+        KernelTimer kt(h, PM_K_CLEAR);
+        PM_HIP(h, pmk::fill_zero(h->rho, (size_t)8 * n * n, h->stream));
+    }
+    {
+        KernelTimer kt(h, PM_K_DEPOSIT);
+        PM_HIP(h, pmk::deposit(n, h->cfg.mode, h->x, h->y, h->m, h->np, std::ldexp(1.0f, h->shift), h->rho,
+                               h->stream));
+    }
+    return PM_OK;
+}
+
+pm_status stage_field(pm_sim* h) {
+    const int n = h->cfg.n_lattice;
+    const bool spectral = h->cfg.force_mode == PM_FORCE_SPECTRAL;
+    {
+        KernelTimer kt(h, PM_K_ROW_R2C);
+        PM_HIP(h, pmk::row_r2c(n, h->rho, std::ldexp(1.0f, -h->shift), h->tw, h->H, h->stream));
+    }
+    {
+        KernelTimer kt(h, PM_K_COL_CONV);
+        PM_HIP(h, pmk::col_conv(n, h->H, h->Hfy, h->T, h->dk, h->tw, spectral, h->stream));
+    }
+    {
+        KernelTimer kt(h, PM_K_ROW_C2R);
+        PM_HIP(h, pmk::row_c2r(n, h->H, h->Hfy, h->dk, h->tw, h->phi, h->force, spectral, h->stream));
+    }
+    if (spectral) {
+        KernelTimer kt(h, PM_K_EDGE);
+        PM_HIP(h, pmk::edge_force(n, h->phi, h->cfg.gravity, h->force, h->stream));
+    } else {
+        KernelTimer kt(h, PM_K_STENCIL);
+        PM_HIP(h, pmk::stencil_force(n, h->phi, h->cfg.gravity, h->force, h->stream));
+    }
+    return PM_OK;
+}
+
+pm_status stage_push(pm_sim* h) {
+    KernelTimer kt(h, PM_K_PUSH);
+    PM_HIP(h, pmk::push(h->cfg.n_lattice, h->cfg.mode, h->force, h->x, h->y, h->vx, h->vy, h->np, h->cfg.dt,
+                        h->cfg.wrap_mode, h->stream));
+    return PM_OK;
+}
+
+int auto_shift(const float* m, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += std::fabs((double)m[i]);
+    if (!(s > 0.0) || !std::isfinite(s)) return 40;
+    // cell sums stay below sum|m| * 2^shift + 2n <= 2^61 + 2^35 < 2^63
+    int sh = (int)std::floor(61.0 - std::log2(s));
+    return std::max(0, std::min(62, sh));
+}
+
+pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, const float* vy, const float* m,
+                 int64_t n) {
+    const size_t b = (size_t)n * 4;
+    PM_HIP(h, hipMemcpyAsync(h->x, x, b, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->y, y, b, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->vx, vx, b, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->vy, vy, b, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->m, m, b, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    h->np = n;
+    h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
+    return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pm_config_default(pm_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->n_lattice = 2048;       // Constants::N (cuh:38)
+    c->mode = PM_CIC;
+    c->n_particles = 0;
+    c->dt = 50.0f;             // Constants::dt (cuh:44)
+    c->gravity = 1.0f;         // Constants::gravityMultiplier (cuh:46)
+    c->green_origin = 3.0;     // cuh:113-116
+    c->deconv_eps = 1e-3;      // cuh:139
+    c->wrap_mode = PM_WRAP_KEEP_VELOCITY;
+    c->force_mode = PM_FORCE_SPECTRAL;
+    c->precision = PM_FFT_FP32;
+    c->mass_shift = 0;
+    c->device = 0;
+}
+
+int64_t pm_instance_bytes(const pm_config* cfg) {
+    std::string why;
+    if (validate(cfg, &why) != PM_OK) return -1;
+    return layout_of(cfg).total;
+}
+
+int pm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+pm_status pm_create(const pm_config* cfg, pm_handle* out) {
+    if (!out) return fail(nullptr, PM_EINVAL, "null output handle");
+    *out = nullptr;
+    std::string why;
+    if (validate(cfg, &why) != PM_OK) return fail(nullptr, PM_EINVAL, why);
+    int ndev = pm_device_count();
+    if (ndev <= 0) return fail(nullptr, PM_ENODEV, "no HIP device visible");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, PM_EINVAL, "device ordinal out of range");
+    if (hipSetDevice(cfg->device) != hipSuccess) return fail(nullptr, PM_EDEVICE, "hipSetDevice failed");
+
+    pm_sim* h = new (std::nothrow) pm_sim();
+    if (!h) return fail(nullptr, PM_ENOMEM, "host allocation failed");
+    h->cfg = *cfg;
+    h->cap = cfg->n_particles;
+    const Layout L = layout_of(cfg);
+    const int n = cfg->n_lattice, nk = n / 2 + 1;
+    pm_status st = PM_OK;
+    auto bail = [&](pm_status s) { release(h); std::string e = h->err; delete h; g_last_error = e; return s; };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->stream = nullptr;
+        return bail(fail(h, PM_EDEVICE, "hipStreamCreate failed"));
+    }
+    void* p;
+    float** parts[5] = {&h->x, &h->y, &h->vx, &h->vy, &h->m};
+    for (auto q : parts) {
+        if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st);
+        *q = static_cast<float*>(p);
+    }
+    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho = static_cast<int64_t*>(p);
+    if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
+    if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->Hfy = static_cast<float2*>(p);
+    if ((st = dalloc(h, &p, L.T)) != PM_OK) return bail(st); h->T = static_cast<float*>(p);
+    if ((st = dalloc(h, &p, L.dk)) != PM_OK) return bail(st); h->dk = static_cast<float*>(p);
+    if ((st = dalloc(h, &p, L.tw)) != PM_OK) return bail(st); h->tw = static_cast<float2*>(p);
+    if ((st = dalloc(h, &p, L.phi)) != PM_OK) return bail(st); h->phi = static_cast<float*>(p);
+    if ((st = dalloc(h, &p, L.force)) != PM_OK) return bail(st); h->force = static_cast<float2*>(p);
+    if ((st = dalloc(h, &p, L.diag)) != PM_OK) return bail(st); h->diag = static_cast<double*>(p);
+
+    // Twiddles exp(-2 pi i m/N) and the stencil symbol d(k) = G (8 sin t - sin 2t)/6,
+    // t = 2 pi k/N (exactly 0 at k = 0 and k = N/2), both evaluated in FP64.
+    std::vector<float2> tw(n);
+    std::vector<float> dk(n);
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < n; ++k) {
+        const double a = 2.0 * pi * k / n;
+        tw[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+        dk[k] = (k == 0 || k == n / 2) ? 0.0f
+                                       : (float)((double)cfg->gravity * (8.0 * std::sin(a) - std::sin(2.0 * a)) / 6.0);
+    }
+    if (hipMemcpyAsync(h->tw, tw.data(), 8 * (size_t)n, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(h->dk, dk.data(), 4 * (size_t)n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return bail(fail(h, PM_EDEVICE, "twiddle upload failed"));
+
+    // FP64 transfer function (setup scratch freed right after)
+    double* work = nullptr;
+    hipError_t e = hipMalloc(&work, (size_t)L.setup_work);
+    if (e != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(setup work)"));
+    e = pmk::build_transfer(n, cfg->mode, cfg->green_origin, cfg->deconv_eps, work, h->T, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(work);
+    if (e != hipSuccess) return bail(hip_fail(h, e, "build_transfer"));
+    if (e2 != hipSuccess) return bail(hip_fail(h, e2, "build_transfer sync"));
+    // deterministic initial lattices (force/potential read before any step are zero)
+    if (pmk::fill_zero(h->phi, (size_t)4 * n * n, h->stream) != hipSuccess ||
+        pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||
+        pmk::fill_zero(h->rho, (size_t)8 * n * n, h->stream) != hipSuccess ||
+        pmk::fill_zero(h->H, (size_t)8 * nk * n, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return bail(fail(h, PM_EDEVICE, "lattice clear failed"));
+    *out = h;
+    return PM_OK;
+}
+
+pm_status pm_destroy(pm_handle h) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    (void)hipSetDevice(h->cfg.device);
+    release(h);
+    delete h;
+    return PM_OK;
+}
+
+pm_status pm_load(pm_handle h, const float* x, const float* y, const float* vx, const float* vy, const float* m,
+                  int64_t n) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (n < 0 || n > h->cap) return fail(h, PM_EINVAL, "particle count exceeds capacity");
+    if (n > 0 && (!x || !y || !vx || !vy || !m)) return fail(h, PM_EINVAL, "null particle array");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    return upload(h, x, y, vx, vy, m, n);
+}
+
+pm_status pm_load_verlet(pm_handle h, const float* x, const float* y, const float* x_old, const float* y_old,
+                         const float* m, int64_t n) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (n < 0 || n > h->cap) return fail(h, PM_EINVAL, "particle count exceeds capacity");
+    if (n > 0 && (!x || !y || !x_old || !y_old || !m)) return fail(h, PM_EINVAL, "null particle array");
+    if (h->cfg.dt == 0.0f) return fail(h, PM_EINVAL, "Verlet state needs dt != 0");
+    std::vector<float> vx((size_t)n), vy((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        vx[i] = (x[i] - x_old[i]) / h->cfg.dt;
+        vy[i] = (y[i] - y_old[i]) / h->cfg.dt;
+    }
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    return upload(h, x, y, vx.data(), vy.data(), m, n);
+}
+
+pm_status pm_stage(pm_handle h, int32_t stage) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    switch (stage) {
+        case PM_STAGE_DEPOSIT: return stage_deposit(h);
+        case PM_STAGE_FIELD: return stage_field(h);
+        case PM_STAGE_PUSH: return stage_push(h);
+        default: return fail(h, PM_EINVAL, "unknown stage");
+    }
+}
+
+pm_status pm_step(pm_handle h, int32_t nsteps) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (nsteps < 0) return fail(h, PM_EINVAL, "nsteps < 0");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    for (int32_t s = 0; s < nsteps; ++s) {
+        pm_status st;
+        if ((st = stage_deposit(h)) != PM_OK) return st;
+        if ((st = stage_field(h)) != PM_OK) return st;
+        if ((st = stage_push(h)) != PM_OK) return st;
+    }
+    return PM_OK;
+}
+
+pm_status pm_sync(pm_handle h) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
+pm_status pm_read(pm_handle h, float* x, float* y, float* vx, float* vy) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    const size_t b = (size_t)h->np * 4;
+    if (b) {
+        if (x) PM_HIP(h, hipMemcpyAsync(x, h->x, b, hipMemcpyDeviceToHost, h->stream));
+        if (y) PM_HIP(h, hipMemcpyAsync(y, h->y, b, hipMemcpyDeviceToHost, h->stream));
+        if (vx) PM_HIP(h, hipMemcpyAsync(vx, h->vx, b, hipMemcpyDeviceToHost, h->stream));
+        if (vy) PM_HIP(h, hipMemcpyAsync(vy, h->vy, b, hipMemcpyDeviceToHost, h->stream));
+    }
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
+pm_status pm_read_verlet(pm_handle h, float* x, float* y, float* x_old, float* y_old) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    std::vector<float> vx((size_t)h->np), vy((size_t)h->np), px((size_t)h->np), py((size_t)h->np);
+    pm_status st = pm_read(h, px.data(), py.data(), vx.data(), vy.data());
+    if (st != PM_OK) return st;
+    for (int64_t i = 0; i < h->np; ++i) {
+        if (x) x[i] = px[i];
+        if (y) y[i] = py[i];
+        if (x_old) x_old[i] = px[i] - vx[i] * h->cfg.dt;
+        if (y_old) y_old[i] = py[i] - vy[i] * h->cfg.dt;
+    }
+    return PM_OK;
+}
+
+pm_status pm_read_lattice(pm_handle h, int32_t which, void* dst, size_t bytes) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (!dst) return fail(h, PM_EINVAL, "null destination");
+    const size_t n = (size_t)h->cfg.n_lattice, nk = n / 2 + 1;
+    const void* src = nullptr;
+    size_t need = 0;
+    switch (which) {
+        case PM_LATTICE_MASS: src = h->rho; need = 8 * n * n; break;
+        case PM_LATTICE_POTENTIAL: src = h->phi; need = 4 * n * n; break;
+        case PM_LATTICE_FORCE: src = h->force; need = 8 * n * n; break;
+        case PM_LATTICE_TRANSFER: src = h->T; need = 4 * nk * n; break;
+        default: return fail(h, PM_EINVAL, "unknown lattice");
+    }
+    if (bytes < need) return fail(h, PM_EINVAL, "destination too small");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    PM_HIP(h, hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
+pm_status pm_diagnostics(pm_handle h, double out[8]) {
+    if (!h || !out) return fail(h, PM_EINVAL, "null argument");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    const int n = h->cfg.n_lattice;
+    const int pb = pmk::diag_blocks(std::max<int64_t>(h->np, 1));
+    std::vector<double> part((size_t)pb * 8, 0.0);
+    if (h->np > 0) {
+        PM_HIP(h, pmk::diag_particles(n, h->x, h->y, h->vx, h->vy, h->m, h->np, h->diag, h->stream));
+        PM_HIP(h, hipMemcpyAsync(part.data(), h->diag, part.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    }
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < (h->np > 0 ? pb : 0); ++b) {
+        for (int q = 0; q < 7; ++q) {
+            if (q == 4) s[4] = std::max(s[4], part[b * 8 + 4]);
+            else s[q] += part[b * 8 + q];
+        }
+    }
+    const int lb = pmk::diag_lattice_blocks(n);
+    std::vector<double> lp((size_t)lb, 0.0);
+    PM_HIP(h, pmk::diag_lattice(n, h->rho, std::ldexp(1.0, -h->shift), h->phi, h->diag, h->stream));
+    PM_HIP(h, hipMemcpyAsync(lp.data(), h->diag, lp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    double pe = 0.0;
+    for (double v : lp) pe += v;
+    out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
+    out[4] = 0.5 * pe; out[5] = s[4]; out[6] = s[5]; out[7] = s[6];
+    return PM_OK;
+}
+
+pm_status pm_get_info(pm_handle h, pm_info* info) {
+    if (!h || !info) return fail(h, PM_EINVAL, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->n_lattice = h->cfg.n_lattice;
+    info->mass_shift = h->shift;
+    info->n_particles = h->np;
+    info->capacity = h->cap;
+    info->device_bytes = h->bytes;
+    info->device = h->cfg.device;
+    return PM_OK;
+}
+
+pm_status pm_profile(pm_handle h, int32_t enable) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    h->profiling = enable != 0;
+    return PM_OK;
+}
+
+pm_status pm_profile_read(pm_handle h, int32_t kernel, double* total_ms, int64_t* launches) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (kernel < 0 || kernel >= PM_K_COUNT) return fail(h, PM_EINVAL, "unknown kernel id");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    pm_status st = resolve_profile(h);
+    if (st != PM_OK) return st;
+    if (total_ms) *total_ms = h->prof_ms[kernel];
+    if (launches) *launches = h->prof_n[kernel];
+    return PM_OK;
+}
+
+pm_status pm_profile_reset(pm_handle h) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    pm_status st = resolve_profile(h);
+    if (st != PM_OK) return st;
+    for (int k = 0; k < PM_K_COUNT; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
+    return PM_OK;
+}
+
+void* pm_stream(pm_handle h) { return h ? (void*)h->stream : nullptr; }
+
+const char* pm_last_error(pm_handle h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+const char* pm_status_string(pm_status s) {
+    switch (s) {
+        case PM_OK: return "ok";
+        case PM_EINVAL: return "invalid argument";
+        case PM_ENOMEM: return "out of device memory";
+        case PM_EDEVICE: return "device error";
+        case PM_EBUSY: return "busy";
+        case PM_ECANCELLED: return "cancelled";
+        case PM_ENODEV: return "no device";
+    }
+    return "unknown status";
+}
+
+}  // extern "C"

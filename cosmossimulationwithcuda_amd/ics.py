@@ -1,0 +1,108 @@
+"""Seeded initial conditions for the benchmark and test workloads.
+
+The reference draws its initial conditions from std::random_device
+(CosmosCuda.cuh:936-937, 1083-1084), so they cannot be reproduced; these
+generators are seeded so the GPU run and the CPU check see identical FP32
+inputs.  Positions are in lattice cells, G is the force multiplier, and the
+circular speed follows the lattice force law a = G M(<r) / r^2 of the 1/r
+potential.  Returns float32 arrays (x, y, vx, vy, m).
+"""
+import numpy as np
+
+
+def _f32(*arrs):
+    return tuple(np.ascontiguousarray(a, dtype=np.float32) for a in arrs)
+
+
+def uniform_square(n, lo, hi, seed, mass=1.0):
+    """n particles uniform in [lo, hi)^2 at rest, equal masses."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(lo, hi, n)
+    y = rng.uniform(lo, hi, n)
+    z = np.zeros(n)
+    return _f32(x, y, z, z, np.full(n, mass))
+
+
+def _disk_radii(rng, n, rd, rmax):
+    out = np.empty(0)
+    while out.size < n:
+        k = int((n - out.size) * 1.1) + 64
+        r = rd * rng.standard_gamma(2.0, k)        # surface density ~ exp(-r/rd): p(r) ~ r exp(-r/rd)
+        out = np.concatenate([out, r[r < rmax]])
+    return out[:n]
+
+
+def _enclosed_fraction(r, rd, rmax):
+    f = lambda s: 1.0 - (1.0 + s / rd) * np.exp(-s / rd)
+    return f(r) / f(rmax)
+
+
+def exponential_disk(n, cx, cy, rd, rmax, seed, gravity=1.0, masses="unit", dispersion=0.02,
+                     bulk_vx=0.0, bulk_vy=0.0, rng=None):
+    """Exponential disk: surface density ~ exp(-r/rd) truncated at rmax, uniform
+    azimuth, circular speed from the enclosed mass, Gaussian velocity dispersion
+    as a fraction of the local circular speed.  masses: "unit" or "lognormal"
+    (mu = 0, sigma = 0.5)."""
+    rng = rng if rng is not None else np.random.default_rng(seed)
+    r = _disk_radii(rng, n, rd, rmax)
+    a = rng.uniform(0.0, 2.0 * np.pi, n)
+    m = np.ones(n) if masses == "unit" else rng.lognormal(0.0, 0.5, n)
+    mtot = m.sum()
+    vc = np.sqrt(gravity * mtot * _enclosed_fraction(r, rd, rmax) / np.maximum(r, 1e-3))
+    ca, sa = np.cos(a), np.sin(a)
+    x = cx + r * ca
+    y = cy + r * sa
+    vx = -vc * sa + dispersion * vc * rng.standard_normal(n) + bulk_vx
+    vy = vc * ca + dispersion * vc * rng.standard_normal(n) + bulk_vy
+    return _f32(x, y, vx, vy, m)
+
+
+def circular_speed(mtot, rd, rmax, r, gravity=1.0):
+    return float(np.sqrt(gravity * mtot * _enclosed_fraction(np.float64(r), rd, rmax) / r))
+
+
+def colliding_disks(n, seed, n_lattice=2048, rd=96.0, centres=((700.0, 1024.0), (1348.0, 1024.0)),
+                    approach=0.3, gravity=1.0):
+    """Two exponential disks (n/2 particles each, unit masses, truncated at 5 rd)
+    approaching along x at +/- approach * v_c(rd)."""
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    rmax = 5.0 * rd
+    parts = []
+    for k, (cx, cy) in enumerate(centres):
+        nk = half if k == 0 else n - half
+        v = approach * circular_speed(nk, rd, rmax, rd, gravity)
+        parts.append(exponential_disk(nk, cx, cy, rd, rmax, None, gravity, "unit", 0.02,
+                                      bulk_vx=v if k == 0 else -v, rng=rng))
+    return tuple(np.concatenate([p[i] for p in parts]) for i in range(5))
+
+
+def single_disk(n, seed, gravity=1.0):
+    """2048^2 single disk: rd = 128 cells, truncated at 900, centre (1024, 1024),
+    lognormal masses, 2% dispersion."""
+    return exponential_disk(n, 1024.0, 1024.0, 128.0, 900.0, seed, gravity, "lognormal", 0.02)
+
+
+def plummer_binary(n, seed, n_lattice=4096, a=64.0, separation=800.0, gravity=1.0):
+    """Two Plummer spheres (projected onto the plane), scale radius a, unit
+    masses, on a circular mutual orbit at the given separation."""
+    rng = np.random.default_rng(seed)
+    c = n_lattice / 2.0
+    half = n // 2
+    mtot = float(n)
+    vorb = np.sqrt(gravity * (mtot / 2.0) / (2.0 * separation))   # each body: M/2 at distance sep, radius sep/2
+    out = []
+    for k, sgn in enumerate((-1.0, 1.0)):
+        nk = half if k == 0 else n - half
+        u = rng.uniform(0.0, 0.99, nk)
+        r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+        r = np.minimum(r, 6.0 * a)
+        ct = rng.uniform(-1.0, 1.0, nk)
+        rp = r * np.sqrt(1.0 - ct ** 2)            # projected radius
+        ph = rng.uniform(0.0, 2.0 * np.pi, nk)
+        x = c + sgn * separation / 2.0 + rp * np.cos(ph)
+        y = c + rp * np.sin(ph)
+        vx = np.zeros(nk)
+        vy = np.full(nk, sgn * vorb)
+        out.append(_f32(x, y, vx, vy, np.ones(nk)))
+    return tuple(np.concatenate([p[i] for p in out]) for i in range(5))

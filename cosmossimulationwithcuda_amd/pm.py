@@ -1,0 +1,239 @@
+"""Python binding of libpm.so (include/pm.h) over ctypes.
+
+Mirrors the compute interface of the reference simulator (CosmosCuda.cuh,
+struct Universe): create with a configuration (Constants + ctor arguments,
+cuh:29-74, 881), load particle arrays, step (worker loop body, cuh:1336-1341),
+read state back.  There is no CPU fallback: every call goes to the HIP library
+and a missing library or device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpm.so")
+
+PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_EBUSY, PM_ECANCELLED, PM_ENODEV = range(7)
+NGP, CIC = 0, 1
+WRAP_KEEP_VELOCITY, WRAP_REFERENCE = 0, 1
+FORCE_SPECTRAL, FORCE_STENCIL = 0, 1
+FFT_FP32, FFT_BF16X3 = 0, 1
+LATTICE_MASS, LATTICE_POTENTIAL, LATTICE_FORCE, LATTICE_TRANSFER = range(4)
+STAGE_DEPOSIT, STAGE_FIELD, STAGE_PUSH = range(3)
+KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push"]
+
+
+class PMError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"pm status {status}: {msg}")
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("n_lattice", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("n_particles", ctypes.c_int64),
+        ("dt", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+        ("green_origin", ctypes.c_double),
+        ("deconv_eps", ctypes.c_double),
+        ("wrap_mode", ctypes.c_int32),
+        ("force_mode", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("mass_shift", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("n_lattice", ctypes.c_int32),
+        ("mass_shift", ctypes.c_int32),
+        ("n_particles", ctypes.c_int64),
+        ("capacity", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+_lib = None
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_H = ctypes.c_void_p
+
+
+def lib():
+    """Load libpm.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m cosmossimulationwithcuda_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pm_config_default.argtypes = [ctypes.POINTER(Config)]
+        L.pm_instance_bytes.argtypes = [ctypes.POINTER(Config)]
+        L.pm_instance_bytes.restype = ctypes.c_int64
+        L.pm_device_count.restype = ctypes.c_int
+        L.pm_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(_H)]
+        L.pm_destroy.argtypes = [_H]
+        L.pm_load.argtypes = [_H, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int64]
+        L.pm_load_verlet.argtypes = [_H, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int64]
+        L.pm_step.argtypes = [_H, ctypes.c_int32]
+        L.pm_stage.argtypes = [_H, ctypes.c_int32]
+        L.pm_sync.argtypes = [_H]
+        L.pm_read.argtypes = [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.pm_read_verlet.argtypes = [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.pm_read_lattice.argtypes = [_H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
+        L.pm_diagnostics.argtypes = [_H, ctypes.POINTER(ctypes.c_double)]
+        L.pm_get_info.argtypes = [_H, ctypes.POINTER(Info)]
+        L.pm_profile.argtypes = [_H, ctypes.c_int32]
+        L.pm_profile_read.argtypes = [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int64)]
+        L.pm_profile_reset.argtypes = [_H]
+        L.pm_stream.argtypes = [_H]
+        L.pm_stream.restype = ctypes.c_void_p
+        L.pm_last_error.argtypes = [_H]
+        L.pm_last_error.restype = ctypes.c_char_p
+        L.pm_status_string.argtypes = [ctypes.c_int]
+        L.pm_status_string.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def device_count():
+    return lib().pm_device_count()
+
+
+def default_config(**kw):
+    c = Config()
+    lib().pm_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+def instance_bytes(cfg):
+    return lib().pm_instance_bytes(ctypes.byref(cfg))
+
+
+def _check(st, h=None):
+    if st != PM_OK:
+        msg = lib().pm_last_error(h).decode(errors="replace")
+        raise PMError(st, msg)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Simulation:
+    """One simulation instance on one GPU (one HIP stream)."""
+
+    def __init__(self, cfg=None, **kw):
+        self.cfg = cfg if cfg is not None else default_config(**kw)
+        self.n = self.cfg.n_lattice
+        h = _H()
+        _check(lib().pm_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self._h = h
+
+    # lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().pm_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # state
+    def load(self, x, y, vx, vy, m):
+        arrs = [_f32(a) for a in (x, y, vx, vy, m)]
+        n = len(arrs[0])
+        if any(len(a) != n for a in arrs):
+            raise ValueError("particle arrays differ in length")
+        _check(lib().pm_load(self._h, *arrs, n), self._h)
+
+    def load_verlet(self, x, y, x_old, y_old, m):
+        arrs = [_f32(a) for a in (x, y, x_old, y_old, m)]
+        _check(lib().pm_load_verlet(self._h, *arrs, len(arrs[0])), self._h)
+
+    def info(self):
+        i = Info()
+        _check(lib().pm_get_info(self._h, ctypes.byref(i)), self._h)
+        return i
+
+    def read(self):
+        n = self.info().n_particles
+        out = [np.empty(n, np.float32) for _ in range(4)]
+        _check(lib().pm_read(self._h, *[a.ctypes.data for a in out]), self._h)
+        return tuple(out)
+
+    def read_verlet(self):
+        n = self.info().n_particles
+        out = [np.empty(n, np.float32) for _ in range(4)]
+        _check(lib().pm_read_verlet(self._h, *[a.ctypes.data for a in out]), self._h)
+        return tuple(out)
+
+    def read_lattice(self, which):
+        n = self.n
+        if which == LATTICE_MASS:
+            a = np.empty((n, n), np.int64)
+        elif which == LATTICE_POTENTIAL:
+            a = np.empty((n, n), np.float32)
+        elif which == LATTICE_FORCE:
+            a = np.empty((n, n, 2), np.float32)
+        elif which == LATTICE_TRANSFER:
+            a = np.empty((n // 2 + 1, n), np.float32)
+        else:
+            raise ValueError(which)
+        _check(lib().pm_read_lattice(self._h, which, a.ctypes.data, a.nbytes), self._h)
+        return a
+
+    # stepping
+    def step(self, k=1):
+        _check(lib().pm_step(self._h, k), self._h)
+
+    def stage(self, s):
+        _check(lib().pm_stage(self._h, s), self._h)
+
+    def sync(self):
+        _check(lib().pm_sync(self._h), self._h)
+
+    def diagnostics(self):
+        out = (ctypes.c_double * 8)()
+        _check(lib().pm_diagnostics(self._h, out), self._h)
+        keys = ["mass", "px", "py", "kinetic", "potential", "max_speed", "nonfinite", "in_bounds"]
+        return dict(zip(keys, list(out)))
+
+    # timing
+    def profile(self, on=True):
+        _check(lib().pm_profile(self._h, 1 if on else 0), self._h)
+
+    def profile_reset(self):
+        _check(lib().pm_profile_reset(self._h), self._h)
+
+    def profile_read(self):
+        res = {}
+        for k, name in enumerate(KERNELS):
+            ms = ctypes.c_double()
+            cnt = ctypes.c_int64()
+            _check(lib().pm_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(cnt)), self._h)
+            res[name] = (ms.value, cnt.value)
+        return res

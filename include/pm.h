@@ -1,0 +1,159 @@
+/*
+ * pm.h -- C ABI of the MI355X particle-mesh gravity engine (libpm.so).
+ *
+ * Drop-in for the compute path of CosmosSimulationWithCuda's header-only
+ * simulator (CosmosCuda.cuh, template struct Universe, cuh:830-1428): load
+ * particle arrays, step, copy positions and velocities out.  Plain C types
+ * and pointers only; every call returns a status (the reference exits the
+ * process on error, cuh:19-27).  One handle = one simulation = one HIP stream
+ * on one device.  A handle is not thread-safe; distinct handles may be used
+ * from distinct threads.
+ *
+ * Each entry point names the reference code it replaces ("cuh" =
+ * CosmosCuda.cuh).
+ */
+#ifndef COSMOS_PM_H
+#define COSMOS_PM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_sim* pm_handle;
+
+typedef enum pm_status {
+    PM_OK = 0,
+    PM_EINVAL = 1,      /* bad argument or configuration */
+    PM_ENOMEM = 2,      /* device allocation failed */
+    PM_EDEVICE = 3,     /* HIP runtime or kernel launch error */
+    PM_EBUSY = 4,       /* resource held (serving: budget or slots exhausted) */
+    PM_ECANCELLED = 5,  /* request cancelled */
+    PM_ENODEV = 6       /* no usable GPU */
+} pm_status;
+
+/* mass assignment: the reference's lowAccuracy flag (cuh:881, 890) */
+enum { PM_NGP = 0, PM_CIC = 1 };
+/* wrap handling at the lattice border (cuh:288-303) */
+enum { PM_WRAP_KEEP_VELOCITY = 0, PM_WRAP_REFERENCE = 1 };
+/* force lattice: spectral interior + reference stencil on the edge bands, or
+ * the reference's stencil (cuh:174-229) on every cell */
+enum { PM_FORCE_SPECTRAL = 0, PM_FORCE_STENCIL = 1 };
+/* FFT arithmetic */
+enum { PM_FFT_FP32 = 0, PM_FFT_BF16X3 = 1 };
+/* lattices readable with pm_read_lattice */
+enum {
+    PM_LATTICE_MASS = 0,      /* int64 fixed point, N*N, index i + j*N       */
+    PM_LATTICE_POTENTIAL = 1, /* float, N*N                                   */
+    PM_LATTICE_FORCE = 2,     /* float2 (Fx, Fy) interleaved, N*N             */
+    PM_LATTICE_TRANSFER = 3   /* float, (N/2+1)*N: T[kx*N+ky] incl. 1/N^2     */
+};
+/* single stages of one step, in order (cuh:1336-1341) */
+enum { PM_STAGE_DEPOSIT = 0, PM_STAGE_FIELD = 1, PM_STAGE_PUSH = 2 };
+/* timed kernels (pm_profile_read) */
+enum {
+    PM_K_CLEAR = 0, PM_K_DEPOSIT = 1, PM_K_ROW_R2C = 2, PM_K_COL_CONV = 3, PM_K_ROW_C2R = 4,
+    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_COUNT = 8
+};
+
+/* Replaces the compile-time namespace Constants (cuh:29-74, main.cu:5-25) and
+ * the ctor arguments (cuh:881) with runtime fields. */
+typedef struct pm_config {
+    int32_t n_lattice;     /* N: power of two, 64..8192 (Constants::N)                */
+    int32_t mode;          /* PM_NGP / PM_CIC                                          */
+    int64_t n_particles;   /* capacity in particles                                    */
+    float dt;              /* time step (Constants::dt)                                */
+    float gravity;         /* force multiplier (Constants::gravityMultiplier)          */
+    double green_origin;   /* G at r = 0: 3.0 in the reference (cuh:113-116)           */
+    double deconv_eps;     /* regulariser of the deconvolution, 0.001 (cuh:139)        */
+    int32_t wrap_mode;     /* PM_WRAP_*                                                */
+    int32_t force_mode;    /* PM_FORCE_*                                               */
+    int32_t precision;     /* PM_FFT_*                                                 */
+    int32_t mass_shift;    /* fixed-point bits (rho = q * 2^-shift); 0 = from the masses */
+    int32_t device;        /* HIP device ordinal                                       */
+    int32_t reserved[7];
+} pm_config;
+
+typedef struct pm_info {
+    int32_t n_lattice;
+    int32_t mass_shift;    /* in effect after the last pm_load */
+    int64_t n_particles;   /* loaded */
+    int64_t capacity;
+    int64_t device_bytes;  /* device memory held by the handle */
+    int32_t device;
+    int32_t reserved[5];
+} pm_info;
+
+/* Defaults of the reference: N 2048, CIC, dt 50, G 1, G(0) 3, eps 0.001. */
+void pm_config_default(pm_config* cfg);
+
+/* Device memory one simulation of this configuration holds (for admission
+ * by memory budget).  -1 on an invalid configuration. */
+int64_t pm_instance_bytes(const pm_config* cfg);
+
+/* Number of HIP devices visible (0 without a GPU). */
+int pm_device_count(void);
+
+/* Allocate lattices and particle arrays, build the FP64 transfer function.
+ * Replaces the device part of Universe::Universe (cuh:969-1035). */
+pm_status pm_create(const pm_config* cfg, pm_handle* out);
+
+/* Free everything.  Replaces Universe::~Universe (cuh:1396-1427). */
+pm_status pm_destroy(pm_handle h);
+
+/* Copy n <= capacity particles from host SoA arrays (caller keeps them).
+ * Replaces the H2D uploads of the ctor / clear / addGalaxy
+ * (cuh:1012-1016, 1053-1060, 1151-1155).  Picks the fixed-point shift from
+ * sum|m| unless cfg.mass_shift is set. */
+pm_status pm_load(pm_handle h, const float* x, const float* y, const float* vx, const float* vy,
+                  const float* m, int64_t n);
+
+/* Same, from the reference's Verlet state (x, xOld): v = (x - xOld)/dt. */
+pm_status pm_load_verlet(pm_handle h, const float* x, const float* y, const float* x_old,
+                         const float* y_old, const float* m, int64_t n);
+
+/* Enqueue nsteps full steps on the handle's stream (asynchronous).  Replaces
+ * the worker loop body (cuh:1336-1341): deposit, FFT convolution, gradient,
+ * gather + update. */
+pm_status pm_step(pm_handle h, int32_t nsteps);
+
+/* Enqueue one stage of a step (PM_STAGE_*), for stage-level checks. */
+pm_status pm_stage(pm_handle h, int32_t stage);
+
+/* Wait for all work on the handle's stream. */
+pm_status pm_sync(pm_handle h);
+
+/* Blocking copy of the state into caller buffers (any pointer may be NULL).
+ * The reference has no read-back path (its device state never returns). */
+pm_status pm_read(pm_handle h, float* x, float* y, float* vx, float* vy);
+
+/* Same, as the reference's Verlet state: x_old = x - v*dt. */
+pm_status pm_read_verlet(pm_handle h, float* x, float* y, float* x_old, float* y_old);
+
+/* Blocking copy of a lattice (PM_LATTICE_*) into dst of at least bytes. */
+pm_status pm_read_lattice(pm_handle h, int32_t which, void* dst, size_t bytes);
+
+/* FP64 reductions on the device: out[0] mass, [1] px, [2] py, [3] kinetic
+ * energy, [4] potential energy 1/2 sum rho*Phi (of the last field solve),
+ * [5] max |v|, [6] non-finite particles, [7] particles in bounds. */
+pm_status pm_diagnostics(pm_handle h, double out[8]);
+
+pm_status pm_get_info(pm_handle h, pm_info* info);
+
+/* Per-kernel device timing with HIP events on the handle's stream. */
+pm_status pm_profile(pm_handle h, int32_t enable);
+pm_status pm_profile_read(pm_handle h, int32_t kernel, double* total_ms, int64_t* launches);
+pm_status pm_profile_reset(pm_handle h);
+
+/* The handle's hipStream_t, for callers that order their own work after it. */
+void* pm_stream(pm_handle h);
+
+const char* pm_last_error(pm_handle h);
+const char* pm_status_string(pm_status s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COSMOS_PM_H */

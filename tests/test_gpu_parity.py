@@ -1,0 +1,252 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars (stated per test):
+  * deposited int64 lattice: bit-exact vs the oracle deposit on identical FP32 positions
+  * stencil band / full stencil force: bit-exact vs the oracle's FP32 stencil on the GPU potential
+  * gather + kick-drift: bit-exact vs the oracle's FP32 push on the GPU force lattice
+  * potential and force: relative L2 <= 1e-5 vs the FP64 oracle on the same lattice
+  * trajectories: per-step position error vs the FP64 oracle stated and bounded
+"""
+import numpy as np
+import pytest
+
+from cosmossimulationwithcuda_amd import ics, pm
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+REL_L2_FIELD = 1e-5          # potential and force vs FP64 (spectral force mode)
+REL_L2_STENCIL_FORCE = 5e-5  # force_mode=stencil: stencil on the FP32 potential amplifies FFT rounding
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def make_sim(n, np_, mode=pm.CIC, **kw):
+    base = dict(n_lattice=n, n_particles=np_, mode=mode, dt=0.01, gravity=1.0, green_origin=1.0,
+                deconv_eps=1e-3)
+    base.update(kw)
+    return pm.Simulation(pm.default_config(**base))
+
+
+def edge_case_particles(n, k, seed):
+    """Particles on cell corners, bounds thresholds, just outside, NaN, and
+    near the wrap thresholds."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, n, k).astype(np.float32)
+    y = rng.uniform(0, n, k).astype(np.float32)
+    special = np.float32([1.0, n - 1.0, np.nextafter(np.float32(n - 1), np.float32(0)), 0.999, 2.0,
+                          2.0001, n - 2.0, n - 2.0001, 17.0, np.nan, -3.0, n + 4.0])
+    x[:len(special)] = special
+    y[:len(special)] = np.float32(n / 2 + 0.25)
+    y[len(special):2 * len(special)] = special
+    return x, y
+
+
+# -------------------------------------------------------------- spectra --
+@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("mode", [pm.NGP, pm.CIC])
+def test_transfer_function_vs_fp64(n, mode):
+    """T built in FP64 on the GPU equals the oracle's literal composite
+    Ghat*conj(Dhat)/(|Dhat|^2+eps) times (-1)^(kx+ky) (the N/2 shift), / N^2."""
+    with make_sim(n, 16, mode, green_origin=3.0) as s:
+        T = s.read_lattice(pm.LATTICE_TRANSFER).astype(np.float64)       # [kx, ky]
+    re, im = orc.transfer(n, mode, 3.0, 1e-3)                            # [ky, kx]
+    k = np.arange(n)
+    sgn = (-1.0) ** (k[:, None] + k[None, :])
+    want = (re * sgn).T[: n // 2 + 1, :] / (n * n)
+    assert np.abs(im).max() < 1e-9 * np.abs(re).max()
+    assert np.allclose(T, want, rtol=2e-7, atol=1e-7 * np.abs(want).max())
+
+
+# -------------------------------------------------------------- deposit --
+@pytest.mark.parametrize("mode", [pm.NGP, pm.CIC])
+def test_deposit_bit_exact_c1(mode):
+    """Config 1 positions (256^2, 65,536 uniform in [64,192)) plus edge cases,
+    lognormal masses: int64 lattice identical to the oracle."""
+    n, npart = 256, 65536
+    x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=1)
+    ex, ey = edge_case_particles(n, 4096, seed=2)
+    x = np.concatenate([x, ex]); y = np.concatenate([y, ey])
+    m = np.random.default_rng(3).lognormal(0, 0.5, len(x)).astype(np.float32)
+    z = np.zeros_like(x)
+    with make_sim(n, len(x), mode) as s:
+        s.load(x, y, z, z, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        got = s.read_lattice(pm.LATTICE_MASS)
+        shift = s.info().mass_shift
+    want = orc.deposit(n, mode, shift, x, y, m)
+    assert np.array_equal(got, want)
+    # permutation invariance on the GPU
+    p = np.random.default_rng(4).permutation(len(x))
+    with make_sim(n, len(x), mode, mass_shift=shift) as s:
+        s.load(x[p], y[p], z, z, m[p])
+        s.stage(pm.STAGE_DEPOSIT)
+        assert np.array_equal(s.read_lattice(pm.LATTICE_MASS), want)
+
+
+def test_deposit_hot_cell_and_empty():
+    """All particles in one cell (maximum contention), and an empty load."""
+    n, npart = 128, 200000
+    x = np.full(npart, 40.5, np.float32); y = np.full(npart, 70.25, np.float32)
+    m = np.random.default_rng(5).uniform(0.5, 1.5, npart).astype(np.float32)
+    z = np.zeros(npart, np.float32)
+    with make_sim(n, npart, pm.CIC) as s:
+        s.load(x, y, z, z, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        got = s.read_lattice(pm.LATTICE_MASS)
+        want = orc.deposit(n, pm.CIC, s.info().mass_shift, x, y, m)
+        assert np.array_equal(got, want)
+        s.load(x[:0], y[:0], z[:0], z[:0], m[:0])
+        s.step(1)
+        assert not s.read_lattice(pm.LATTICE_MASS).any()
+
+
+# ---------------------------------------------------------------- field --
+@pytest.mark.parametrize("n,mode", [(256, pm.CIC), (256, pm.NGP), (1024, pm.CIC)])
+def test_field_vs_fp64(n, mode):
+    npart = 65536 if n == 256 else 400000
+    lo, hi = n // 4, 3 * n // 4
+    x, y, vx, vy, m = ics.uniform_square(npart, lo, hi, seed=n + mode)
+    with make_sim(n, npart, mode) as s:
+        s.load(x, y, vx, vy, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        s.stage(pm.STAGE_FIELD)
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        phi = s.read_lattice(pm.LATTICE_POTENTIAL)
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        shift = s.info().mass_shift
+    T = orc.transfer(n, mode, 1.0, 1e-3)
+    phi64 = orc.potential(n, T, shift, rho)
+    fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+    e_phi = rel_l2(phi, phi64)
+    e_f = float(np.sqrt(np.sum((force[..., 0] - fx64) ** 2 + (force[..., 1] - fy64) ** 2)
+                        / np.sum(fx64 ** 2 + fy64 ** 2)))
+    print(f"N={n} mode={mode}: potential rel L2 {e_phi:.3e}, force rel L2 {e_f:.3e}")
+    assert e_phi <= REL_L2_FIELD
+    assert e_f <= REL_L2_FIELD
+    # edge bands: bit-exact reference stencil on the GPU potential
+    f32 = orc.gradient32(n, 1.0, phi)
+    for sl in (np.s_[:, :2], np.s_[:, n - 2:]):
+        assert np.array_equal(force[sl][..., 0], f32[sl][..., 0])
+    for sl in (np.s_[:2, :], np.s_[n - 2:, :]):
+        assert np.array_equal(force[sl][..., 1], f32[sl][..., 1])
+
+
+def test_stencil_force_mode():
+    """force_mode=stencil: the reference stencil on every cell, bit-exact vs the
+    oracle's FP32 stencil on the GPU potential; within 5e-5 of FP64."""
+    n, npart = 256, 65536
+    x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=9)
+    with make_sim(n, npart, pm.CIC, force_mode=pm.FORCE_STENCIL) as s:
+        s.load(x, y, vx, vy, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        s.stage(pm.STAGE_FIELD)
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        phi = s.read_lattice(pm.LATTICE_POTENTIAL)
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        shift = s.info().mass_shift
+    assert np.array_equal(force, orc.gradient32(n, 1.0, phi))
+    phi64 = orc.potential(n, orc.transfer(n, pm.CIC, 1.0, 1e-3), shift, rho)
+    fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+    e_f = float(np.sqrt(np.sum((force[..., 0] - fx64) ** 2 + (force[..., 1] - fy64) ** 2)
+                        / np.sum(fx64 ** 2 + fy64 ** 2)))
+    print(f"stencil mode force rel L2 {e_f:.3e}")
+    assert e_f <= REL_L2_STENCIL_FORCE
+
+
+# ----------------------------------------------------------------- push --
+@pytest.mark.parametrize("mode", [pm.NGP, pm.CIC])
+@pytest.mark.parametrize("wrap", [pm.WRAP_KEEP_VELOCITY, pm.WRAP_REFERENCE])
+def test_push_bit_exact(mode, wrap):
+    n, npart = 256, 50000
+    rng = np.random.default_rng(10 + mode + 2 * wrap)
+    x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=11)
+    ex, ey = edge_case_particles(n, 2000, seed=12)
+    x = np.concatenate([x, ex]); y = np.concatenate([y, ey])
+    vx = rng.normal(0, 40, len(x)).astype(np.float32)    # large speeds drive wraps at the borders
+    vy = rng.normal(0, 40, len(x)).astype(np.float32)
+    m = np.ones(len(x), np.float32)
+    with make_sim(n, len(x), mode, wrap_mode=wrap, dt=0.05) as s:
+        s.load(x, y, vx, vy, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        s.stage(pm.STAGE_FIELD)
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        s.stage(pm.STAGE_PUSH)
+        got = s.read()
+    want = orc.push32(n, mode, np.float32(0.05), wrap, x, y, vx, vy, force)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint32), w.view(np.uint32))
+
+
+def test_verlet_roundtrip():
+    n, npart = 128, 1000
+    x, y, _, _, m = ics.uniform_square(npart, 30, 90, seed=13)
+    rng = np.random.default_rng(14)
+    xo = (x - rng.normal(0, 0.1, npart)).astype(np.float32)
+    yo = (y - rng.normal(0, 0.1, npart)).astype(np.float32)
+    with make_sim(n, npart, pm.CIC, dt=0.5) as s:
+        s.load_verlet(x, y, xo, yo, m)
+        _, _, vx, vy = s.read()
+        assert np.allclose(vx, (x - xo) / np.float32(0.5), rtol=1e-6)
+        x2, y2, xo2, yo2 = s.read_verlet()
+        assert np.array_equal(x2, x) and np.allclose(xo2, xo, atol=1e-5)
+
+
+# ---------------------------------------------------------- trajectories --
+def test_config1_trajectory_vs_fp64():
+    """Config 1: 256^2, 65,536 particles uniform in [64,192)^2 at rest, unit
+    mass, CIC, G(0)=1, G=1, dt=0.01, 100 steps.  Per-step position error vs the
+    FP64 oracle trajectory and vs the CPU FP32-state implementation."""
+    n, npart, dt, steps = 256, 65536, 0.01, 100
+    x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=1)
+    ref64 = orc.Step64(n, pm.CIC, dt, 1.0, 1.0, 1e-3, 0, x, y, vx, vy, m)
+    errs = []
+    with make_sim(n, npart, pm.CIC, dt=dt) as s:
+        s.load(x, y, vx, vy, m)
+        mix = orc.MixedStepper(n, pm.CIC, np.float32(dt), 1.0, 1.0, 1e-3, 0, s.info().mass_shift,
+                               x, y, vx, vy, m)
+        for k in range(1, steps + 1):
+            s.step(1)
+            ref64.step(1)
+            if k % 10 == 0 or k == 1:
+                gx, gy, gvx, gvy = s.read()
+                d = np.hypot(gx - ref64.x, gy - ref64.y)
+                errs.append((k, float(d.max()), float(np.sqrt(np.mean(d * d)))))
+        mix.step(steps)
+        gx, gy, _, _ = s.read()
+    for k, mx, rms in errs:
+        print(f"step {k:3d}: |x_gpu - x_fp64| max {mx:.3e} rms {rms:.3e} cells")
+    d_mix = np.hypot(gx - mix.x, gy - mix.y)
+    print(f"step {steps}: |x_gpu - x_cpu32| max {d_mix.max():.3e} rms {np.sqrt(np.mean(d_mix ** 2)):.3e}")
+    assert errs[-1][1] < 1e-2 and errs[-1][2] < 1e-3
+    assert d_mix.max() < 1e-2
+
+
+def test_diagnostics_vs_numpy():
+    n, npart = 256, 30000
+    x, y, vx, vy, m = ics.exponential_disk(npart, 128, 128, 20, 90, seed=15, masses="lognormal")
+    with make_sim(n, npart, pm.CIC) as s:
+        s.load(x, y, vx, vy, m)
+        s.step(1)
+        d = s.diagnostics()
+        gx, gy, gvx, gvy = s.read()
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        phi = s.read_lattice(pm.LATTICE_POTENTIAL)
+        shift = s.info().mass_shift
+    m64 = m.astype(np.float64)
+    assert np.isclose(d["mass"], m64.sum(), rtol=1e-12)
+    assert np.isclose(d["px"], np.sum(m64 * gvx), rtol=1e-9, atol=1e-9)
+    assert np.isclose(d["kinetic"], 0.5 * np.sum(m64 * (gvx.astype(np.float64) ** 2 + gvy.astype(np.float64) ** 2)),
+                      rtol=1e-9)
+    assert np.isclose(d["potential"], 0.5 * np.sum(rho * 2.0 ** -shift * phi.astype(np.float64)), rtol=1e-9)
+    assert d["nonfinite"] == 0 and d["in_bounds"] == npart
+
+
+def test_invalid_load_rejected():
+    with make_sim(64, 10) as s:
+        z = np.zeros(11, np.float32)
+        with pytest.raises(pm.PMError) as ei:
+            s.load(z, z, z, z, z)
+        assert ei.value.status == pm.PM_EINVAL

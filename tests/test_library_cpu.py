@@ -1,0 +1,67 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol that
+include/pm.h declares, validates configurations before touching a device, and
+reports errors as status codes (no compute without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from cosmossimulationwithcuda_amd import pm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pm.lib()
+    names = declared_functions("pm.h")
+    assert "pm_create" in names and "pm_step" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_config_defaults_match_reference_constants():
+    c = pm.default_config()
+    assert c.n_lattice == 2048 and c.mode == pm.CIC
+    assert c.dt == 50.0 and c.gravity == 1.0
+    assert c.green_origin == 3.0 and c.deconv_eps == 1e-3
+
+
+def test_instance_bytes_and_validation():
+    c = pm.default_config(n_lattice=2048, n_particles=100_000_000)
+    b = pm.instance_bytes(c)
+    assert b >= 5 * 4 * 100_000_000 + 8 * 2048 * 2048
+    assert b < 5 * 4 * 100_000_000 + 64 * 2048 * 2048
+    assert pm.instance_bytes(pm.default_config(n_lattice=1000)) == -1
+    assert pm.instance_bytes(pm.default_config(mode=7)) == -1
+    assert pm.instance_bytes(pm.default_config(mass_shift=63)) == -1
+
+
+def test_create_rejects_invalid_config_before_device():
+    h = ctypes.c_void_p()
+    bad = pm.default_config(n_lattice=100, n_particles=10)
+    assert pm.lib().pm_create(ctypes.byref(bad), ctypes.byref(h)) == pm.PM_EINVAL
+    assert b"power of two" in pm.lib().pm_last_error(None)
+
+
+@pytest.mark.skipif(pm.device_count() > 0, reason="checks the no-device path")
+def test_create_without_device_reports_enodev():
+    h = ctypes.c_void_p()
+    c = pm.default_config(n_lattice=64, n_particles=10)
+    assert pm.lib().pm_create(ctypes.byref(c), ctypes.byref(h)) == pm.PM_ENODEV
+    with pytest.raises(pm.PMError):
+        pm.Simulation(c)
+
+
+def test_null_handle_calls_return_einval():
+    L = pm.lib()
+    assert L.pm_step(None, 1) == pm.PM_EINVAL
+    assert L.pm_sync(None) == pm.PM_EINVAL
+    assert L.pm_destroy(None) == pm.PM_EINVAL
+    assert L.pm_status_string(pm.PM_ENOMEM) == b"out of device memory"

@@ -8,7 +8,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
+#include <utility>
 #include <new>
 #include <string>
 #include <vector>
@@ -32,7 +34,14 @@ struct pm_sim {
     hipStream_t stream = nullptr;
     int64_t cap = 0, np = 0;
     int shift = 40;
-    float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *m = nullptr;
+    pmk::ParticleArrays cur{}, alt{};      // sorted state and the counting-sort target / read scratch
+    pmk::BinState bins{};
+    int nbins = 0;                         // in-bounds bins + parked bin
+    int max_chunks = 0;
+    int resort_interval = 4;
+    int steps_since_sort = 0;
+    int64_t sorts = 0;
+    unsigned long long* migrants = nullptr;
     int64_t* rho = nullptr;
     float2 *H = nullptr, *Hfy = nullptr;
     float* T = nullptr;
@@ -76,7 +85,8 @@ pm_status validate(const pm_config* c, std::string* why) {
     if (!c) { *why = "null config"; return PM_EINVAL; }
     if (!pmk::lattice_supported(c->n_lattice)) { *why = "n_lattice must be a power of two in [64, 8192]"; return PM_EINVAL; }
     if (c->mode != PM_NGP && c->mode != PM_CIC) { *why = "mode must be PM_NGP or PM_CIC"; return PM_EINVAL; }
-    if (c->n_particles < 0 || c->n_particles > ((int64_t)1 << 34)) { *why = "n_particles out of range"; return PM_EINVAL; }
+    if (c->n_particles < 0 || c->n_particles > (int64_t)INT32_MAX) { *why = "n_particles must be in [0, 2^31-1]"; return PM_EINVAL; }
+    if (c->resort_interval < 0) { *why = "resort_interval must be >= 0"; return PM_EINVAL; }
     if (!std::isfinite(c->dt) || !std::isfinite(c->gravity)) { *why = "dt and gravity must be finite"; return PM_EINVAL; }
     if (!(c->deconv_eps >= 0.0) || !std::isfinite(c->green_origin)) { *why = "bad deconv_eps or green_origin"; return PM_EINVAL; }
     if (c->wrap_mode != PM_WRAP_KEEP_VELOCITY && c->wrap_mode != PM_WRAP_REFERENCE) { *why = "bad wrap_mode"; return PM_EINVAL; }
@@ -90,13 +100,15 @@ pm_status validate(const pm_config* c, std::string* why) {
 int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
 struct Layout {
-    int64_t particles, rho, H, T, dk, tw, phi, force, diag, total, setup_work;
+    int64_t particles, bins, rho, H, T, dk, tw, phi, force, diag, total, setup_work;
 };
 
 Layout layout_of(const pm_config* c) {
     const int64_t n = c->n_lattice, nk = n / 2 + 1, cap = std::max<int64_t>(c->n_particles, 1);
     Layout L{};
     L.particles = align256(4 * cap);
+    const int64_t nb = pmk::bin_count_of((int)n);
+    L.bins = align256(4 * (nb + 1)) * 3 + align256(8 * (nb + 2)) + 256;
     L.rho = align256(8 * n * n);
     L.H = align256(8 * nk * n);
     L.T = align256(4 * nk * n);
@@ -106,7 +118,7 @@ Layout layout_of(const pm_config* c) {
     L.force = align256(8 * n * n);
     const int64_t dblocks = std::max(pmk::diag_blocks(cap), pmk::diag_lattice_blocks((int)n));
     L.diag = align256(8 * 8 * dblocks);
-    L.total = 5 * L.particles + L.rho + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force + L.diag;
+    L.total = 12 * L.particles + L.bins + L.rho + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force + L.diag;
     L.setup_work = 16 * n * n;
     return L;
 }
@@ -168,16 +180,29 @@ pm_status resolve_profile(pm_sim* h) {
 }
 
 // ---- stages -----------------------------------------------------------------
+pm_status sort_now(pm_sim* h) {
+    KernelTimer kt(h, PM_K_SORT);
+    PM_HIP(h, pmk::sort_particles(h->cfg.n_lattice, h->cur, h->alt, h->np, h->bins, h->stream));
+    std::swap(h->cur, h->alt);
+    h->steps_since_sort = 0;
+    h->sorts += 1;
+    return PM_OK;
+}
+
 pm_status stage_deposit(pm_sim* h) {
     const int n = h->cfg.n_lattice;
+    if (h->steps_since_sort >= h->resort_interval) {
+        pm_status st = sort_now(h);
+        if (st != PM_OK) return st;
+    }
     {
         KernelTimer kt(h, PM_K_CLEAR);
         PM_HIP(h, pmk::fill_zero(h->rho, (size_t)8 * n * n, h->stream));
     }
     {
         KernelTimer kt(h, PM_K_DEPOSIT);
-        PM_HIP(h, pmk::deposit(n, h->cfg.mode, h->x, h->y, h->m, h->np, std::ldexp(1.0f, h->shift), h->rho,
-                               h->stream));
+        PM_HIP(h, pmk::deposit(n, h->cfg.mode, h->cur.x, h->cur.y, h->cur.m, h->bins, h->max_chunks,
+                               std::ldexp(1.0f, h->shift), h->rho, h->migrants, h->stream));
     }
     return PM_OK;
 }
@@ -209,8 +234,9 @@ pm_status stage_field(pm_sim* h) {
 
 pm_status stage_push(pm_sim* h) {
     KernelTimer kt(h, PM_K_PUSH);
-    PM_HIP(h, pmk::push(h->cfg.n_lattice, h->cfg.mode, h->force, h->x, h->y, h->vx, h->vy, h->np, h->cfg.dt,
-                        h->cfg.wrap_mode, h->stream));
+    PM_HIP(h, pmk::push(h->cfg.n_lattice, h->cfg.mode, h->force, h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, h->np,
+                        h->cfg.dt, h->cfg.wrap_mode, h->stream));
+    h->steps_since_sort += 1;
     return PM_OK;
 }
 
@@ -226,13 +252,18 @@ int auto_shift(const float* m, int64_t n) {
 pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, const float* vy, const float* m,
                  int64_t n) {
     const size_t b = (size_t)n * 4;
-    PM_HIP(h, hipMemcpyAsync(h->x, x, b, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipMemcpyAsync(h->y, y, b, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipMemcpyAsync(h->vx, vx, b, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipMemcpyAsync(h->vy, vy, b, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipMemcpyAsync(h->m, m, b, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipStreamSynchronize(h->stream));
+    if (b) {
+        PM_HIP(h, hipMemcpyAsync(h->cur.x, x, b, hipMemcpyHostToDevice, h->stream));
+        PM_HIP(h, hipMemcpyAsync(h->cur.y, y, b, hipMemcpyHostToDevice, h->stream));
+        PM_HIP(h, hipMemcpyAsync(h->cur.vx, vx, b, hipMemcpyHostToDevice, h->stream));
+        PM_HIP(h, hipMemcpyAsync(h->cur.vy, vy, b, hipMemcpyHostToDevice, h->stream));
+        PM_HIP(h, hipMemcpyAsync(h->cur.m, m, b, hipMemcpyHostToDevice, h->stream));
+    }
     h->np = n;
+    PM_HIP(h, pmk::iota_ids(h->cur.id, n, h->stream));
+    pm_status st = sort_now(h);
+    if (st != PM_OK) return st;
+    PM_HIP(h, hipStreamSynchronize(h->stream));
     h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
     return PM_OK;
 }
@@ -293,10 +324,26 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
         return bail(fail(h, PM_EDEVICE, "hipStreamCreate failed"));
     }
     void* p;
-    float** parts[5] = {&h->x, &h->y, &h->vx, &h->vy, &h->m};
+    float** parts[10] = {&h->cur.x, &h->cur.y, &h->cur.vx, &h->cur.vy, &h->cur.m,
+                         &h->alt.x, &h->alt.y, &h->alt.vx, &h->alt.vy, &h->alt.m};
     for (auto q : parts) {
         if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st);
         *q = static_cast<float*>(p);
+    }
+    if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->cur.id = static_cast<uint32_t*>(p);
+    if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->alt.id = static_cast<uint32_t*>(p);
+    h->nbins = pmk::bin_count_of(n);
+    h->max_chunks = pmk::deposit_max_chunks(n, h->cap);
+    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 4;
+    if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
+    {
+        char* bp = static_cast<char*>(p);
+        const int64_t ib = align256(4 * (int64_t)(h->nbins + 1));
+        h->bins.count = reinterpret_cast<int*>(bp);
+        h->bins.chunk_start = reinterpret_cast<int*>(bp + ib);
+        h->bins.cursor = reinterpret_cast<int*>(bp + 2 * ib);
+        h->bins.start = reinterpret_cast<int64_t*>(bp + 3 * ib);
+        h->migrants = reinterpret_cast<unsigned long long*>(bp + 3 * ib + align256(8 * (int64_t)(h->nbins + 2)));
     }
     if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho = static_cast<int64_t*>(p);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
@@ -337,6 +384,7 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
         pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->rho, (size_t)8 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->H, (size_t)8 * nk * n, h->stream) != hipSuccess ||
+        pmk::fill_zero(h->bins.count, (size_t)L.bins, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(fail(h, PM_EDEVICE, "lattice clear failed"));
     *out = h;
@@ -411,10 +459,14 @@ pm_status pm_read(pm_handle h, float* x, float* y, float* vx, float* vy) {
     PM_HIP(h, hipSetDevice(h->cfg.device));
     const size_t b = (size_t)h->np * 4;
     if (b) {
-        if (x) PM_HIP(h, hipMemcpyAsync(x, h->x, b, hipMemcpyDeviceToHost, h->stream));
-        if (y) PM_HIP(h, hipMemcpyAsync(y, h->y, b, hipMemcpyDeviceToHost, h->stream));
-        if (vx) PM_HIP(h, hipMemcpyAsync(vx, h->vx, b, hipMemcpyDeviceToHost, h->stream));
-        if (vy) PM_HIP(h, hipMemcpyAsync(vy, h->vy, b, hipMemcpyDeviceToHost, h->stream));
+        // slots hold particles in bin order; scatter them back to load order in the scratch set
+        PM_HIP(h, pmk::unpermute(h->cur.id, h->np, h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, x ? h->alt.x : nullptr,
+                                 y ? h->alt.y : nullptr, vx ? h->alt.vx : nullptr, vy ? h->alt.vy : nullptr,
+                                 h->stream));
+        if (x) PM_HIP(h, hipMemcpyAsync(x, h->alt.x, b, hipMemcpyDeviceToHost, h->stream));
+        if (y) PM_HIP(h, hipMemcpyAsync(y, h->alt.y, b, hipMemcpyDeviceToHost, h->stream));
+        if (vx) PM_HIP(h, hipMemcpyAsync(vx, h->alt.vx, b, hipMemcpyDeviceToHost, h->stream));
+        if (vy) PM_HIP(h, hipMemcpyAsync(vy, h->alt.vy, b, hipMemcpyDeviceToHost, h->stream));
     }
     PM_HIP(h, hipStreamSynchronize(h->stream));
     return PM_OK;
@@ -461,7 +513,8 @@ pm_status pm_diagnostics(pm_handle h, double out[8]) {
     const int pb = pmk::diag_blocks(std::max<int64_t>(h->np, 1));
     std::vector<double> part((size_t)pb * 8, 0.0);
     if (h->np > 0) {
-        PM_HIP(h, pmk::diag_particles(n, h->x, h->y, h->vx, h->vy, h->m, h->np, h->diag, h->stream));
+        PM_HIP(h, pmk::diag_particles(n, h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, h->cur.m, h->np, h->diag,
+                                      h->stream));
         PM_HIP(h, hipMemcpyAsync(part.data(), h->diag, part.size() * 8, hipMemcpyDeviceToHost, h->stream));
     }
     PM_HIP(h, hipStreamSynchronize(h->stream));
@@ -493,6 +546,12 @@ pm_status pm_get_info(pm_handle h, pm_info* info) {
     info->capacity = h->cap;
     info->device_bytes = h->bytes;
     info->device = h->cfg.device;
+    info->sorts = h->sorts;
+    unsigned long long mig = 0;
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    PM_HIP(h, hipMemcpyAsync(&mig, h->migrants, sizeof(mig), hipMemcpyDeviceToHost, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    info->migrants = (int64_t)mig;
     return PM_OK;
 }
 

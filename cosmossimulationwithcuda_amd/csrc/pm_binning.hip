@@ -1,0 +1,199 @@
+// Spatial ordering of the particle arrays by lattice tile.
+//
+// Particles are kept physically sorted by the 64x64-cell tile ("bin") that
+// holds them, refreshed by a counting sort every few steps.  A deposit block
+// then owns a contiguous chunk of one bin and accumulates into an LDS window,
+// and the force gather of consecutive particles hits the same few cache lines.
+// The order of particles inside a bin is arbitrary (atomic slot claims); it
+// does not affect results because the deposit is an integer sum and every
+// other kernel is per particle.  id[slot] keeps each particle's load index so
+// reads return the caller's order.
+//
+// Bins: b = (y/64) * (N/64) + x/64 for in-bounds particles (1 <= x,y < N-1,
+// the reference's rule, cuh:257, 497), b = nbins for the rest ("parked": never
+// deposited, never moved).
+#include "pm_internal.h"
+
+namespace pmk {
+namespace {
+
+__device__ __forceinline__ int bin_of(float x, float y, int n, int tpr) {
+    const float hi = (float)(n - 1);
+    if (!(x >= 1.0f && x < hi && y >= 1.0f && y < hi)) return tpr * tpr;
+    return ((int)y >> kTileLog) * tpr + ((int)x >> kTileLog);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_count(const float* __restrict__ x, const float* __restrict__ y,
+                                                       int64_t np, int n, int* __restrict__ count) {
+    extern __shared__ int hist[];
+    const int tpr = n >> kTileLog, nb = tpr * tpr + 1;
+    for (int i = threadIdx.x; i < nb; i += kBlock) hist[i] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < np; p += stride)
+        atomicAdd(&hist[bin_of(x[p], y[p], n, tpr)], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kBlock)
+        if (hist[i]) atomicAdd(&count[i], hist[i]);
+}
+
+// One block: exclusive scans of the bin counts (-> start[0..nb]) and of the
+// deposit chunk counts of the in-bounds bins (-> chunk_start[0..nb-1]).
+// Zeroes the scatter cursors.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan(const int* __restrict__ count, int nb, int chunk,
+                                                          int64_t* __restrict__ start, int* __restrict__ chunk_start,
+                                                          int* __restrict__ cursor) {
+    __shared__ int64_t s_cnt[kScanBlock];
+    __shared__ int s_chk[kScanBlock];
+    const int t = threadIdx.x;
+    const int per = (nb + kScanBlock - 1) / kScanBlock;
+    const int b0 = t * per, b1 = min(nb, b0 + per);
+    int64_t c = 0;
+    int k = 0;
+    for (int b = b0; b < b1; ++b) {
+        c += count[b];
+        if (b < nb - 1) k += (int)(((int64_t)count[b] + chunk - 1) / chunk);
+    }
+    s_cnt[t] = c;
+    s_chk[t] = k;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {      // Hillis-Steele inclusive scan
+        const int64_t a = t >= off ? s_cnt[t - off] : 0;
+        const int ak = t >= off ? s_chk[t - off] : 0;
+        __syncthreads();
+        s_cnt[t] += a;
+        s_chk[t] += ak;
+        __syncthreads();
+    }
+    int64_t run = s_cnt[t] - c;
+    int runk = s_chk[t] - k;
+    for (int b = b0; b < b1; ++b) {
+        start[b] = run;
+        chunk_start[b] = runk;
+        cursor[b] = 0;
+        run += count[b];
+        if (b < nb - 1) runk += (int)(((int64_t)count[b] + chunk - 1) / chunk);
+    }
+    if (t == kScanBlock - 1) start[nb] = s_cnt[t];
+}
+
+// Counting-sort scatter: each block takes kScatterPer particles per thread,
+// claims per-bin ranges with one global atomic per (block, bin), and moves
+// x, y, vx, vy, m, id to their sorted slots in the second buffer set.
+constexpr int kScatterPer = 8;
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(ParticleArrays src, ParticleArrays dst, int64_t np, int n,
+                                                         const int64_t* __restrict__ start,
+                                                         int* __restrict__ cursor) {
+    extern __shared__ int lds[];
+    const int tpr = n >> kTileLog, nb = tpr * tpr + 1;
+    int* lcnt = lds;
+    int* lbase = lds + nb;
+    for (int i = threadIdx.x; i < nb; i += kBlock) lcnt[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kBlock * kScatterPer;
+    int bq[kScatterPer], rq[kScatterPer];
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q) {
+        const int64_t p = base + q * kBlock + threadIdx.x;
+        bq[q] = -1;
+        if (p < np) {
+            bq[q] = bin_of(src.x[p], src.y[p], n, tpr);
+            rq[q] = atomicAdd(&lcnt[bq[q]], 1);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q)
+        if (bq[q] >= 0 && rq[q] == 0) lbase[bq[q]] = atomicAdd(&cursor[bq[q]], lcnt[bq[q]]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q) {
+        const int64_t p = base + q * kBlock + threadIdx.x;
+        if (bq[q] < 0) continue;
+        const int64_t s = start[bq[q]] + lbase[bq[q]] + rq[q];
+        dst.x[s] = src.x[p];
+        dst.y[s] = src.y[p];
+        dst.vx[s] = src.vx[p];
+        dst.vy[s] = src.vy[p];
+        dst.m[s] = src.m[p];
+        dst.id[s] = src.id[p];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* __restrict__ id, int64_t np) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < np; p += stride) id[p] = (uint32_t)p;
+}
+
+// out[id[s]] = in[s] for up to four arrays (null pointers skipped).
+__global__ __launch_bounds__(kBlock) void k_unpermute(const uint32_t* __restrict__ id, int64_t np,
+                                                       const float* __restrict__ a0, const float* __restrict__ a1,
+                                                       const float* __restrict__ a2, const float* __restrict__ a3,
+                                                       float* __restrict__ o0, float* __restrict__ o1,
+                                                       float* __restrict__ o2, float* __restrict__ o3) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < np; s += stride) {
+        const uint32_t d = id[s];
+        if (o0) o0[d] = a0[s];
+        if (o1) o1[d] = a1[s];
+        if (o2) o2[d] = a2[s];
+        if (o3) o3[d] = a3[s];
+    }
+}
+
+// Dynamic LDS above 64 KiB (large lattices: 16,385 bins) must be allowed per kernel.
+hipError_t allow_lds(const void* fn, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int grid_for(int64_t np, int per_thread, int cap) {
+    int64_t b = (np + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+}  // namespace
+
+int bin_count_of(int n) {
+    const int tpr = n >> kTileLog;
+    return tpr * tpr + 1;
+}
+
+hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, hipStream_t s) {
+    const int nb = bin_count_of(n);
+    hipError_t e = fill_zero(bs.count, sizeof(int) * (size_t)nb, s);
+    if (e != hipSuccess) return e;
+    if ((e = allow_lds(reinterpret_cast<const void*>(&k_bin_count), sizeof(int) * nb)) != hipSuccess) return e;
+    if ((e = allow_lds(reinterpret_cast<const void*>(&k_bin_scatter), 2 * sizeof(int) * nb)) != hipSuccess) return e;
+    if (np > 0) {
+        hipLaunchKernelGGL(k_bin_count, dim3(grid_for(np, 64, 2048)), dim3(kBlock), sizeof(int) * nb, s, cur.x, cur.y,
+                           np, n, bs.count);
+    }
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, kDepositChunk, bs.start,
+                       bs.chunk_start, bs.cursor);
+    if (np > 0) {
+        const int64_t blocks = (np + (int64_t)kBlock * kScatterPer - 1) / ((int64_t)kBlock * kScatterPer);
+        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(kBlock), 2 * sizeof(int) * nb, s, cur, alt, np,
+                           n, bs.start, bs.cursor);
+    }
+    return hipGetLastError();
+}
+
+hipError_t iota_ids(uint32_t* id, int64_t np, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(np, 4, 4096)), dim3(kBlock), 0, s, id, np);
+    return hipGetLastError();
+}
+
+hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const float* a1, const float* a2,
+                     const float* a3, float* o0, float* o1, float* o2, float* o3, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpermute, dim3(grid_for(np, 4, 4096)), dim3(kBlock), 0, s, id, np, a0, a1, a2, a3, o0, o1,
+                       o2, o3);
+    return hipGetLastError();
+}
+
+}  // namespace pmk

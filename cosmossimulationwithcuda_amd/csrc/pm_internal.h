@@ -8,16 +8,50 @@ namespace pmk {
 
 constexpr int kBlock = 256;   // threads per block for every streaming kernel (4 wave64s)
 
+// Particle binning: 64x64-cell tiles; a deposit block accumulates one chunk of
+// one tile into an LDS window of the tile plus a kHalo-cell margin (plus the
+// +1 column/row CIC reaches).
+constexpr int kTileLog = 6;
+constexpr int kTile = 1 << kTileLog;
+constexpr int kHalo = 7;
+constexpr int kWin = kTile + 2 * kHalo + 1;          // 79: 49,928 B of int64 per window
+constexpr int kDepositBlock = 512;
+constexpr int kDepositChunk = 16384;                 // particles per deposit block
+
+struct ParticleArrays {
+    float *x, *y, *vx, *vy, *m;
+    uint32_t* id;   // load index of the particle in each slot
+};
+
+struct BinState {
+    int* count;          // [nbins + 1] particles per bin (last = parked)
+    int64_t* start;      // [nbins + 2] exclusive prefix of count
+    int* chunk_start;    // [nbins + 1] exclusive prefix of deposit chunks per in-bounds bin
+    int* cursor;         // [nbins + 1] scatter cursors
+};
+
 // Lattice sizes with compiled kernels (powers of two).
 inline bool lattice_supported(int n) {
     return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 ||
            n == 8192;
 }
 
-// ----- deposit (pm_deposit.hip) ------------------------------------------------
-// rho must be zero on entry; adds int64 fixed-point contributions round(w*m*scale).
-hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, int64_t np,
-                   float scale, int64_t* rho, hipStream_t s);
+// ----- binning (pm_binning.hip) -----------------------------------------------
+int bin_count_of(int n);   // in-bounds bins + 1 parked bin
+// Counting sort of cur into alt by bin; fills bs.  Caller swaps cur and alt.
+hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, hipStream_t s);
+hipError_t iota_ids(uint32_t* id, int64_t np, hipStream_t s);
+hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const float* a1, const float* a2,
+                     const float* a3, float* o0, float* o1, float* o2, float* o3, hipStream_t s);
+
+// ----- deposit (pm_particles.hip) ---------------------------------------------
+// rho must be zero on entry; adds int64 fixed-point contributions round(w*m*scale)
+// for the particles of every in-bounds bin (bs from the last sort_particles).
+// max_chunks: upper bound of the chunk count (grid size).  migrants counts the
+// contributions made outside the tile window.
+int deposit_max_chunks(int n, int64_t capacity);
+hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
+                   float scale, int64_t* rho, unsigned long long* migrants, hipStream_t s);
 
 // ----- FFT field solve (pm_fft.hip) -------------------------------------------
 // Half spectrum layout: H[k * N + j], k in [0, N/2] (x frequency), j in [0, N)

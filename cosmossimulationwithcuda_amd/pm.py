@@ -21,7 +21,7 @@ FORCE_SPECTRAL, FORCE_STENCIL = 0, 1
 FFT_FP32, FFT_BF16X3 = 0, 1
 LATTICE_MASS, LATTICE_POTENTIAL, LATTICE_FORCE, LATTICE_TRANSFER = range(4)
 STAGE_DEPOSIT, STAGE_FIELD, STAGE_PUSH = range(3)
-KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push"]
+KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push", "sort"]
 
 
 class PMError(RuntimeError):
@@ -44,7 +44,8 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("mass_shift", ctypes.c_int32),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("resort_interval", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -56,7 +57,9 @@ class Info(ctypes.Structure):
         ("capacity", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("reserved", ctypes.c_int32),
+        ("sorts", ctypes.c_int64),
+        ("migrants", ctypes.c_int64),
     ]
 
 

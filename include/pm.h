@@ -55,7 +55,7 @@ enum { PM_STAGE_DEPOSIT = 0, PM_STAGE_FIELD = 1, PM_STAGE_PUSH = 2 };
 /* timed kernels (pm_profile_read) */
 enum {
     PM_K_CLEAR = 0, PM_K_DEPOSIT = 1, PM_K_ROW_R2C = 2, PM_K_COL_CONV = 3, PM_K_ROW_C2R = 4,
-    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_COUNT = 8
+    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_SORT = 8, PM_K_COUNT = 9
 };
 
 /* Replaces the compile-time namespace Constants (cuh:29-74, main.cu:5-25) and
@@ -73,7 +73,9 @@ typedef struct pm_config {
     int32_t precision;     /* PM_FFT_*                                                 */
     int32_t mass_shift;    /* fixed-point bits (rho = q * 2^-shift); 0 = from the masses */
     int32_t device;        /* HIP device ordinal                                       */
-    int32_t reserved[7];
+    int32_t resort_interval; /* steps between re-binning the particles by lattice tile;
+                                0 = default (4).  Affects speed only, never results.    */
+    int32_t reserved[6];
 } pm_config;
 
 typedef struct pm_info {
@@ -83,7 +85,9 @@ typedef struct pm_info {
     int64_t capacity;
     int64_t device_bytes;  /* device memory held by the handle */
     int32_t device;
-    int32_t reserved[5];
+    int32_t reserved;
+    int64_t sorts;         /* particle re-binnings performed since creation            */
+    int64_t migrants;      /* deposits that fell outside their tile window (global atomics) */
 } pm_info;
 
 /* Defaults of the reference: N 2048, CIC, dt 50, G 1, G(0) 3, eps 0.001. */

@@ -36,8 +36,9 @@ def test_config_defaults_match_reference_constants():
 def test_instance_bytes_and_validation():
     c = pm.default_config(n_lattice=2048, n_particles=100_000_000)
     b = pm.instance_bytes(c)
-    assert b >= 5 * 4 * 100_000_000 + 8 * 2048 * 2048
-    assert b < 5 * 4 * 100_000_000 + 64 * 2048 * 2048
+    # two SoA sets (x, y, vx, vy, m, id) for the binning sort: 48 B per particle
+    assert b >= 48 * 100_000_000 + 8 * 2048 * 2048
+    assert b < 48 * 100_000_000 + 64 * 2048 * 2048
     assert pm.instance_bytes(pm.default_config(n_lattice=1000)) == -1
     assert pm.instance_bytes(pm.default_config(mode=7)) == -1
     assert pm.instance_bytes(pm.default_config(mass_shift=63)) == -1

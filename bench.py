@@ -49,6 +49,7 @@ def algorithmic_bytes(kernel, n, np_):
         "edge": 8 * 8 * n,
         "stencil": 4 * n * n + 8 * n * n,
         "push": 32 * np_ + 8 * n * n,             # read/write x, y, vx, vy; read force lattice
+        "sort": 0,                                # re-binning is implementation traffic, not algorithmic
     }[kernel]
 
 
@@ -135,7 +136,7 @@ def main():
         avg_ms = ms / cnt
         gb = algorithmic_bytes(name, n, np_) / 1e9
         kernels[name] = {"avg_ms": round(avg_ms, 5), "launches": cnt, "algorithmic_GB": round(gb, 4),
-                         "GB_s": round(gb / (avg_ms / 1e3), 1)}
+                         "GB_s": round(gb / (avg_ms / 1e3), 1) if avg_ms > 0 else None}
     dom = max(("deposit", "push"), key=lambda k: kernels.get(k, {"avg_ms": 0})["avg_ms"])
     kd = kernels[dom]
     traffic = None

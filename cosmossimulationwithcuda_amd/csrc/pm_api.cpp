@@ -333,7 +333,8 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->cur.id = static_cast<uint32_t*>(p);
     if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->alt.id = static_cast<uint32_t*>(p);
     h->nbins = pmk::bin_count_of(n);
-    h->max_chunks = pmk::deposit_max_chunks(n, h->cap);
+    h->bins.chunk = pmk::kDepositChunkDefault;
+    h->max_chunks = pmk::deposit_max_chunks(n, h->cap, h->bins.chunk);
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 4;
     if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
     {
@@ -535,6 +536,26 @@ pm_status pm_diagnostics(pm_handle h, double out[8]) {
     out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
     out[4] = 0.5 * pe; out[5] = s[4]; out[6] = s[5]; out[7] = s[6];
     return PM_OK;
+}
+
+pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
+    if (!h || !name) return fail(h, PM_EINVAL, "null argument");
+    const std::string k(name);
+    if (k == "resort_interval") {
+        if (value < 1 || value > (1 << 20)) return fail(h, PM_EINVAL, "resort_interval must be in [1, 2^20]");
+        h->resort_interval = (int)value;
+        return PM_OK;
+    }
+    if (k == "deposit_chunk") {
+        if (value < pmk::kDepositBlock || value > (1 << 24) || (value % pmk::kDepositBlock) != 0)
+            return fail(h, PM_EINVAL, "deposit_chunk must be a multiple of 512 in [512, 2^24]");
+        // chunk starts depend on the chunk size: re-bin on the next deposit
+        h->bins.chunk = (int)value;
+        h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->cap, h->bins.chunk);
+        h->steps_since_sort = h->resort_interval;
+        return PM_OK;
+    }
+    return fail(h, PM_EINVAL, "unknown parameter: " + k);
 }
 
 pm_status pm_get_info(pm_handle h, pm_info* info) {

@@ -172,7 +172,7 @@ hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t
         hipLaunchKernelGGL(k_bin_count, dim3(grid_for(np, 64, 2048)), dim3(kBlock), sizeof(int) * nb, s, cur.x, cur.y,
                            np, n, bs.count);
     }
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, kDepositChunk, bs.start,
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk, bs.start,
                        bs.chunk_start, bs.cursor);
     if (np > 0) {
         const int64_t blocks = (np + (int64_t)kBlock * kScatterPer - 1) / ((int64_t)kBlock * kScatterPer);

@@ -16,7 +16,7 @@ constexpr int kTile = 1 << kTileLog;
 constexpr int kHalo = 7;
 constexpr int kWin = kTile + 2 * kHalo + 1;          // 79: 49,928 B of int64 per window
 constexpr int kDepositBlock = 512;
-constexpr int kDepositChunk = 16384;                 // particles per deposit block
+constexpr int kDepositChunkDefault = 65536;          // particles per deposit block (tunable)
 
 struct ParticleArrays {
     float *x, *y, *vx, *vy, *m;
@@ -28,6 +28,7 @@ struct BinState {
     int64_t* start;      // [nbins + 2] exclusive prefix of count
     int* chunk_start;    // [nbins + 1] exclusive prefix of deposit chunks per in-bounds bin
     int* cursor;         // [nbins + 1] scatter cursors
+    int chunk;           // particles per deposit block
 };
 
 // Lattice sizes with compiled kernels (powers of two).
@@ -49,7 +50,7 @@ hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const floa
 // for the particles of every in-bounds bin (bs from the last sort_particles).
 // max_chunks: upper bound of the chunk count (grid size).  migrants counts the
 // contributions made outside the tile window.
-int deposit_max_chunks(int n, int64_t capacity);
+int deposit_max_chunks(int n, int64_t capacity, int chunk);
 hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
                    float scale, int64_t* rho, unsigned long long* migrants, hipStream_t s);
 

@@ -33,7 +33,8 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit_bin(const float* __re
                                                                const float* __restrict__ y,
                                                                const float* __restrict__ m, int n,
                                                                const int64_t* __restrict__ start,
-                                                               const int* __restrict__ chunk_start, float scale,
+                                                               const int* __restrict__ chunk_start, int chunk,
+                                                               float scale,
                                                                unsigned long long* __restrict__ rho,
                                                                unsigned long long* __restrict__ migrants) {
     __shared__ unsigned long long win[kWin * kWin];
@@ -48,8 +49,8 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit_bin(const float* __re
         if (chunk_start[mid] <= c) lo = mid + 1; else hi = mid;
     }
     const int b = lo - 1;
-    const int64_t s0 = start[b] + (int64_t)(c - chunk_start[b]) * kDepositChunk;
-    const int64_t s1 = min(s0 + kDepositChunk, start[b + 1]);
+    const int64_t s0 = start[b] + (int64_t)(c - chunk_start[b]) * chunk;
+    const int64_t s1 = min(s0 + chunk, start[b + 1]);
     const int wx0 = (b % tpr) * kTile - kHalo, wy0 = (b / tpr) * kTile - kHalo;
     for (int e = threadIdx.x; e < kWin * kWin; e += kDepositBlock) win[e] = 0ull;
     if (threadIdx.x == 0) s_mig = 0;
@@ -110,59 +111,108 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit_bin(const float* __re
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }
 
+// One particle: gather the force at (x, y) and kick-drift it.  Operation
+// order is fixed (no contraction) so the CPU mirror reproduces it bit for bit.
 template <int MODE, int WRAP>
-__global__ __launch_bounds__(kBlock) void k_push(const float2* __restrict__ force, float* __restrict__ x,
-                                                  float* __restrict__ y, float* __restrict__ vx,
-                                                  float* __restrict__ vy, int64_t np, int n, float dt) {
-    const float hi = (float)(n - 1);
-    const float span = (float)(n - 4);
-    const float lo_wrap = 2.0f, hi_wrap = (float)(n - 2);
-    const float jump = span / dt;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < np; p += stride) {
-        const float px = x[p], py = y[p];
-        if (!in_bounds(px, py, hi)) continue;
-        const int xi = (int)px, yi = (int)py;
-        const size_t c = (size_t)yi * n + xi;
-        float ax, ay;
-        if (MODE == 0) {
-            const float2 f = force[c];
-            ax = f.x;
-            ay = f.y;
-        } else {
-            const float fx = px - (float)xi, fy = py - (float)yi;
-            const float wx = 1.0f - fx, wy = 1.0f - fy;
-            const float w00 = wx * wy, w10 = fx * wy, w01 = wx * fy, w11 = fx * fy;
-            const float2 f00 = force[c], f10 = force[c + 1], f01 = force[c + n], f11 = force[c + n + 1];
-            ax = ((f00.x * w00 + f10.x * w10) + f01.x * w01) + f11.x * w11;
-            ay = ((f00.y * w00 + f10.y * w10) + f01.y * w01) + f11.y * w11;
-        }
-        float nvx = fmaf(ax, dt, vx[p]);
-        float nvy = fmaf(ay, dt, vy[p]);
-        float nx = fmaf(nvx, dt, px);
-        float ny = fmaf(nvy, dt, py);
-        if (nx < lo_wrap) { nx += span; if (WRAP) nvx += jump; }
-        if (ny < lo_wrap) { ny += span; if (WRAP) nvy += jump; }
-        if (nx > hi_wrap) { nx -= span; if (WRAP) nvx -= jump; }
-        if (ny > hi_wrap) { ny -= span; if (WRAP) nvy -= jump; }
-        x[p] = nx;
-        y[p] = ny;
-        vx[p] = nvx;
-        vy[p] = nvy;
+__device__ __forceinline__ void push_one(const float2* __restrict__ force, int n, float dt, float& px, float& py,
+                                         float& vx, float& vy) {
+    if (!in_bounds(px, py, (float)(n - 1))) return;
+    const int xi = (int)px, yi = (int)py;
+    const size_t c = (size_t)yi * n + xi;
+    float ax, ay;
+    if (MODE == 0) {
+        const float2 f = force[c];
+        ax = f.x;
+        ay = f.y;
+    } else {
+        const float fx = px - (float)xi, fy = py - (float)yi;
+        const float wx = 1.0f - fx, wy = 1.0f - fy;
+        const float w00 = wx * wy, w10 = fx * wy, w01 = wx * fy, w11 = fx * fy;
+        const float2 f00 = force[c], f10 = force[c + 1], f01 = force[c + n], f11 = force[c + n + 1];
+        ax = ((f00.x * w00 + f10.x * w10) + f01.x * w01) + f11.x * w11;
+        ay = ((f00.y * w00 + f10.y * w10) + f01.y * w01) + f11.y * w11;
     }
+    const float span = (float)(n - 4);
+    float nvx = fmaf(ax, dt, vx);
+    float nvy = fmaf(ay, dt, vy);
+    float nx = fmaf(nvx, dt, px);
+    float ny = fmaf(nvy, dt, py);
+    if (nx < 2.0f) { nx += span; if (WRAP) nvx += span / dt; }
+    if (ny < 2.0f) { ny += span; if (WRAP) nvy += span / dt; }
+    if (nx > (float)(n - 2)) { nx -= span; if (WRAP) nvx -= span / dt; }
+    if (ny > (float)(n - 2)) { ny -= span; if (WRAP) nvy -= span / dt; }
+    px = nx;
+    py = ny;
+    vx = nvx;
+    vy = nvy;
 }
 
-int particle_grid(int64_t np) {
-    const int64_t blocks = (np + kBlock - 1) / kBlock;
-    const int64_t cap = 256 * 64;   // 256 CUs x 64 blocks in flight is plenty for a grid-stride loop
-    return (int)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks));
+// Four consecutive particles per lane: 16-byte nontemporal loads and stores
+// of x, y, vx, vy (the streams are touched once per step; keeping them out of
+// the caches leaves L2 to the force lattice the gathers reuse).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int MODE, int WRAP>
+__global__ __launch_bounds__(kBlock) void k_push4(const float2* __restrict__ force, v4f* __restrict__ x,
+                                                   v4f* __restrict__ y, v4f* __restrict__ vx,
+                                                   v4f* __restrict__ vy, int64_t n4, int n, float dt) {
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n4) return;
+    v4f X = __builtin_nontemporal_load(x + q);
+    v4f Y = __builtin_nontemporal_load(y + q);
+    v4f VX = __builtin_nontemporal_load(vx + q);
+    v4f VY = __builtin_nontemporal_load(vy + q);
+    float px[4] = {X.x, X.y, X.z, X.w}, py[4] = {Y.x, Y.y, Y.z, Y.w};
+    float ux[4] = {VX.x, VX.y, VX.z, VX.w}, uy[4] = {VY.x, VY.y, VY.z, VY.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) push_one<MODE, WRAP>(force, n, dt, px[k], py[k], ux[k], uy[k]);
+    X = v4f{px[0], px[1], px[2], px[3]};
+    Y = v4f{py[0], py[1], py[2], py[3]};
+    VX = v4f{ux[0], ux[1], ux[2], ux[3]};
+    VY = v4f{uy[0], uy[1], uy[2], uy[3]};
+    __builtin_nontemporal_store(X, x + q);
+    __builtin_nontemporal_store(Y, y + q);
+    __builtin_nontemporal_store(VX, vx + q);
+    __builtin_nontemporal_store(VY, vy + q);
+}
+
+// Remainder particles [first, np) one per lane.
+template <int MODE, int WRAP>
+__global__ __launch_bounds__(kBlock) void k_push_tail(const float2* __restrict__ force, float* __restrict__ x,
+                                                       float* __restrict__ y, float* __restrict__ vx,
+                                                       float* __restrict__ vy, int64_t first, int64_t np, int n,
+                                                       float dt) {
+    const int64_t p = first + threadIdx.x;
+    if (p >= np) return;
+    float px = x[p], py = y[p], ux = vx[p], uy = vy[p];
+    push_one<MODE, WRAP>(force, n, dt, px, py, ux, uy);
+    x[p] = px;
+    y[p] = py;
+    vx[p] = ux;
+    vy[p] = uy;
+}
+
+template <int MODE, int WRAP>
+hipError_t push_t(const float2* force, float* x, float* y, float* vx, float* vy, int64_t np, int n, float dt,
+                  hipStream_t s) {
+    const int64_t n4 = np / 4;
+    if (n4 > 0) {
+        const int64_t blocks = (n4 + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL((k_push4<MODE, WRAP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, force,
+                           reinterpret_cast<v4f*>(x), reinterpret_cast<v4f*>(y), reinterpret_cast<v4f*>(vx),
+                           reinterpret_cast<v4f*>(vy), n4, n, dt);
+    }
+    if (np > 4 * n4)
+        hipLaunchKernelGGL((k_push_tail<MODE, WRAP>), dim3(1), dim3(kBlock), 0, s, force, x, y, vx, vy, 4 * n4, np, n,
+                           dt);
+    return hipGetLastError();
 }
 
 }  // namespace
 
-int deposit_max_chunks(int n, int64_t capacity) {
+int deposit_max_chunks(int n, int64_t capacity, int chunk) {
     const int64_t nin = (int64_t)(n >> kTileLog) * (n >> kTileLog);
-    return (int)((capacity + kDepositChunk - 1) / kDepositChunk + nin);
+    return (int)((capacity + chunk - 1) / chunk + nin);
 }
 
 hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
@@ -171,26 +221,20 @@ hipError_t deposit(int n, int mode, const float* x, const float* y, const float*
     const dim3 grid(max_chunks);
     if (mode == 0)
         hipLaunchKernelGGL((k_deposit_bin<0>), grid, dim3(kDepositBlock), 0, s, x, y, m, n, bs.start,
-                           bs.chunk_start, scale, r, migrants);
+                           bs.chunk_start, bs.chunk, scale, r, migrants);
     else
         hipLaunchKernelGGL((k_deposit_bin<1>), grid, dim3(kDepositBlock), 0, s, x, y, m, n, bs.start,
-                           bs.chunk_start, scale, r, migrants);
+                           bs.chunk_start, bs.chunk, scale, r, migrants);
     return hipGetLastError();
 }
 
 hipError_t push(int n, int mode, const float2* force, float* x, float* y, float* vx, float* vy, int64_t np,
                 float dt, int wrap_mode, hipStream_t s) {
     if (np <= 0) return hipSuccess;
-    const dim3 grid(particle_grid(np));
-    if (mode == 0 && wrap_mode == 0)
-        hipLaunchKernelGGL((k_push<0, 0>), grid, dim3(kBlock), 0, s, force, x, y, vx, vy, np, n, dt);
-    else if (mode == 0)
-        hipLaunchKernelGGL((k_push<0, 1>), grid, dim3(kBlock), 0, s, force, x, y, vx, vy, np, n, dt);
-    else if (wrap_mode == 0)
-        hipLaunchKernelGGL((k_push<1, 0>), grid, dim3(kBlock), 0, s, force, x, y, vx, vy, np, n, dt);
-    else
-        hipLaunchKernelGGL((k_push<1, 1>), grid, dim3(kBlock), 0, s, force, x, y, vx, vy, np, n, dt);
-    return hipGetLastError();
+    if (mode == 0 && wrap_mode == 0) return push_t<0, 0>(force, x, y, vx, vy, np, n, dt, s);
+    if (mode == 0) return push_t<0, 1>(force, x, y, vx, vy, np, n, dt, s);
+    if (wrap_mode == 0) return push_t<1, 0>(force, x, y, vx, vy, np, n, dt, s);
+    return push_t<1, 1>(force, x, y, vx, vy, np, n, dt, s);
 }
 
 }  // namespace pmk

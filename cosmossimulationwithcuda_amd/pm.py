@@ -91,6 +91,7 @@ def lib():
         L.pm_read_lattice.argtypes = [_H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
         L.pm_diagnostics.argtypes = [_H, ctypes.POINTER(ctypes.c_double)]
         L.pm_get_info.argtypes = [_H, ctypes.POINTER(Info)]
+        L.pm_set_param.argtypes = [_H, ctypes.c_char_p, ctypes.c_int64]
         L.pm_profile.argtypes = [_H, ctypes.c_int32]
         L.pm_profile_read.argtypes = [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64)]
@@ -176,6 +177,9 @@ class Simulation:
     def load_verlet(self, x, y, x_old, y_old, m):
         arrs = [_f32(a) for a in (x, y, x_old, y_old, m)]
         _check(lib().pm_load_verlet(self._h, *arrs, len(arrs[0])), self._h)
+
+    def set_param(self, name, value):
+        _check(lib().pm_set_param(self._h, name.encode(), int(value)), self._h)
 
     def info(self):
         i = Info()

@@ -146,6 +146,12 @@ pm_status pm_diagnostics(pm_handle h, double out[8]);
 
 pm_status pm_get_info(pm_handle h, pm_info* info);
 
+/* Runtime tunables that change speed, never results: "resort_interval"
+ * (steps between particle re-binnings) and "deposit_chunk" (particles per
+ * deposit block, a multiple of 512).  No counterpart in the reference, whose
+ * launch shape is the compile-time Constants::THREADS (cuh:31). */
+pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
+
 /* Per-kernel device timing with HIP events on the handle's stream. */
 pm_status pm_profile(pm_handle h, int32_t enable);
 pm_status pm_profile_read(pm_handle h, int32_t kernel, double* total_ms, int64_t* launches);

@@ -50,6 +50,8 @@ def algorithmic_bytes(kernel, n, np_):
         "stencil": 4 * n * n + 8 * n * n,
         "push": 32 * np_ + 8 * n * n,             # read/write x, y, vx, vy; read force lattice
         "sort": 0,                                # re-binning is implementation traffic, not algorithmic
+        # fused push + next deposit: read x, y, vx, vy, m; write x, y, vx, vy; read force; write int64 lattice
+        "push_deposit": 36 * np_ + 16 * n * n,
     }[kernel]
 
 
@@ -137,7 +139,7 @@ def main():
         gb = algorithmic_bytes(name, n, np_) / 1e9
         kernels[name] = {"avg_ms": round(avg_ms, 5), "launches": cnt, "algorithmic_GB": round(gb, 4),
                          "GB_s": round(gb / (avg_ms / 1e3), 1) if avg_ms > 0 else None}
-    dom = max(("deposit", "push"), key=lambda k: kernels.get(k, {"avg_ms": 0})["avg_ms"])
+    dom = max(("deposit", "push", "push_deposit"), key=lambda k: kernels.get(k, {"avg_ms": 0})["avg_ms"])
     kd = kernels[dom]
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")

@@ -38,11 +38,14 @@ struct pm_sim {
     pmk::BinState bins{};
     int nbins = 0;                         // in-bounds bins + parked bin
     int max_chunks = 0;
-    int resort_interval = 4;
+    int resort_interval = 8;
     int steps_since_sort = 0;
     int64_t sorts = 0;
     unsigned long long* migrants = nullptr;
-    int64_t* rho = nullptr;
+    int64_t* rho = nullptr;                // mass lattice of the current positions (when rho_valid)
+    int64_t* rho_next = nullptr;           // target of the fused push + deposit
+    const int64_t* rho_field = nullptr;    // lattice the last field solve consumed (potential energy)
+    bool rho_valid = false;
     float2 *H = nullptr, *Hfy = nullptr;
     float* T = nullptr;
     float* dk = nullptr;
@@ -118,7 +121,7 @@ Layout layout_of(const pm_config* c) {
     L.force = align256(8 * n * n);
     const int64_t dblocks = std::max(pmk::diag_blocks(cap), pmk::diag_lattice_blocks((int)n));
     L.diag = align256(8 * 8 * dblocks);
-    L.total = 12 * L.particles + L.bins + L.rho + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force + L.diag;
+    L.total = 12 * L.particles + L.bins + 2 * L.rho + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force + L.diag;
     L.setup_work = 16 * n * n;
     return L;
 }
@@ -180,9 +183,9 @@ pm_status resolve_profile(pm_sim* h) {
 }
 
 // ---- stages -----------------------------------------------------------------
-pm_status sort_now(pm_sim* h) {
+pm_status sort_now(pm_sim* h, bool counted) {
     KernelTimer kt(h, PM_K_SORT);
-    PM_HIP(h, pmk::sort_particles(h->cfg.n_lattice, h->cur, h->alt, h->np, h->bins, h->stream));
+    PM_HIP(h, pmk::sort_particles(h->cfg.n_lattice, h->cur, h->alt, h->np, h->bins, counted, h->stream));
     std::swap(h->cur, h->alt);
     h->steps_since_sort = 0;
     h->sorts += 1;
@@ -192,7 +195,7 @@ pm_status sort_now(pm_sim* h) {
 pm_status stage_deposit(pm_sim* h) {
     const int n = h->cfg.n_lattice;
     if (h->steps_since_sort >= h->resort_interval) {
-        pm_status st = sort_now(h);
+        pm_status st = sort_now(h, false);
         if (st != PM_OK) return st;
     }
     {
@@ -204,6 +207,7 @@ pm_status stage_deposit(pm_sim* h) {
         PM_HIP(h, pmk::deposit(n, h->cfg.mode, h->cur.x, h->cur.y, h->cur.m, h->bins, h->max_chunks,
                                std::ldexp(1.0f, h->shift), h->rho, h->migrants, h->stream));
     }
+    h->rho_valid = true;
     return PM_OK;
 }
 
@@ -213,6 +217,7 @@ pm_status stage_field(pm_sim* h) {
     {
         KernelTimer kt(h, PM_K_ROW_R2C);
         PM_HIP(h, pmk::row_r2c(n, h->rho, std::ldexp(1.0f, -h->shift), h->tw, h->H, h->stream));
+        h->rho_field = h->rho;
     }
     {
         KernelTimer kt(h, PM_K_COL_CONV);
@@ -237,6 +242,30 @@ pm_status stage_push(pm_sim* h) {
     PM_HIP(h, pmk::push(h->cfg.n_lattice, h->cfg.mode, h->force, h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, h->np,
                         h->cfg.dt, h->cfg.wrap_mode, h->stream));
     h->steps_since_sort += 1;
+    h->rho_valid = false;
+    return PM_OK;
+}
+
+// Step tail with the next deposit fused into the push; the mass lattice of
+// the new positions is left in h->rho for the next field solve.
+pm_status stage_push_deposit(pm_sim* h) {
+    const int n = h->cfg.n_lattice;
+    const bool due = h->steps_since_sort + 1 >= h->resort_interval;
+    {
+        KernelTimer kt(h, PM_K_CLEAR);
+        PM_HIP(h, pmk::fill_zero(h->rho_next, (size_t)8 * n * n, h->stream));
+        if (due) PM_HIP(h, pmk::fill_zero(h->bins.count, sizeof(int) * (size_t)h->nbins, h->stream));
+    }
+    {
+        KernelTimer kt(h, PM_K_PUSH_DEPOSIT);
+        PM_HIP(h, pmk::push_deposit(n, h->cfg.mode, h->cfg.wrap_mode, h->force, h->cur, h->bins, h->max_chunks,
+                                    h->cfg.dt, std::ldexp(1.0f, h->shift), h->rho_next, h->migrants, due,
+                                    h->stream));
+    }
+    std::swap(h->rho, h->rho_next);
+    h->rho_valid = true;
+    h->steps_since_sort += 1;
+    if (due) return sort_now(h, true);
     return PM_OK;
 }
 
@@ -261,7 +290,9 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
     }
     h->np = n;
     PM_HIP(h, pmk::iota_ids(h->cur.id, n, h->stream));
-    pm_status st = sort_now(h);
+    h->rho_valid = false;
+    h->rho_field = nullptr;
+    pm_status st = sort_now(h, false);
     if (st != PM_OK) return st;
     PM_HIP(h, hipStreamSynchronize(h->stream));
     h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
@@ -335,7 +366,7 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     h->nbins = pmk::bin_count_of(n);
     h->bins.chunk = pmk::kDepositChunkDefault;
     h->max_chunks = pmk::deposit_max_chunks(n, h->cap, h->bins.chunk);
-    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 4;
+    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
     if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
     {
         char* bp = static_cast<char*>(p);
@@ -347,6 +378,7 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
         h->migrants = reinterpret_cast<unsigned long long*>(bp + 3 * ib + align256(8 * (int64_t)(h->nbins + 2)));
     }
     if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho = static_cast<int64_t*>(p);
+    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_next = static_cast<int64_t*>(p);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->Hfy = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.T)) != PM_OK) return bail(st); h->T = static_cast<float*>(p);
@@ -441,9 +473,9 @@ pm_status pm_step(pm_handle h, int32_t nsteps) {
     PM_HIP(h, hipSetDevice(h->cfg.device));
     for (int32_t s = 0; s < nsteps; ++s) {
         pm_status st;
-        if ((st = stage_deposit(h)) != PM_OK) return st;
+        if (!h->rho_valid && (st = stage_deposit(h)) != PM_OK) return st;
         if ((st = stage_field(h)) != PM_OK) return st;
-        if ((st = stage_push(h)) != PM_OK) return st;
+        if ((st = stage_push_deposit(h)) != PM_OK) return st;
     }
     return PM_OK;
 }
@@ -528,11 +560,13 @@ pm_status pm_diagnostics(pm_handle h, double out[8]) {
     }
     const int lb = pmk::diag_lattice_blocks(n);
     std::vector<double> lp((size_t)lb, 0.0);
-    PM_HIP(h, pmk::diag_lattice(n, h->rho, std::ldexp(1.0, -h->shift), h->phi, h->diag, h->stream));
-    PM_HIP(h, hipMemcpyAsync(lp.data(), h->diag, lp.size() * 8, hipMemcpyDeviceToHost, h->stream));
-    PM_HIP(h, hipStreamSynchronize(h->stream));
     double pe = 0.0;
-    for (double v : lp) pe += v;
+    if (h->rho_field) {
+        PM_HIP(h, pmk::diag_lattice(n, h->rho_field, std::ldexp(1.0, -h->shift), h->phi, h->diag, h->stream));
+        PM_HIP(h, hipMemcpyAsync(lp.data(), h->diag, lp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        PM_HIP(h, hipStreamSynchronize(h->stream));
+        for (double v : lp) pe += v;
+    }
     out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
     out[4] = 0.5 * pe; out[5] = s[4]; out[6] = s[5]; out[7] = s[6];
     return PM_OK;

@@ -17,12 +17,6 @@
 namespace pmk {
 namespace {
 
-__device__ __forceinline__ int bin_of(float x, float y, int n, int tpr) {
-    const float hi = (float)(n - 1);
-    if (!(x >= 1.0f && x < hi && y >= 1.0f && y < hi)) return tpr * tpr;
-    return ((int)y >> kTileLog) * tpr + ((int)x >> kTileLog);
-}
-
 __global__ __launch_bounds__(kBlock) void k_bin_count(const float* __restrict__ x, const float* __restrict__ y,
                                                        int64_t np, int n, int* __restrict__ count) {
     extern __shared__ int hist[];
@@ -39,14 +33,18 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(const float* __restrict__ 
 
 // One block: exclusive scans of the bin counts (-> start[0..nb]) and of the
 // deposit chunk counts of the in-bounds bins (-> chunk_start[0..nb-1]).
-// Zeroes the scatter cursors.
+// Zeroes the scatter cursors.  add_parked: the counts came from the fused
+// push, which never visits the parked range; add that range's standing size
+// (from the previous start[]) to the parked bin first.
 constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_bin_scan(const int* __restrict__ count, int nb, int chunk,
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count, int nb, int chunk, int add_parked,
                                                           int64_t* __restrict__ start, int* __restrict__ chunk_start,
                                                           int* __restrict__ cursor) {
     __shared__ int64_t s_cnt[kScanBlock];
     __shared__ int s_chk[kScanBlock];
     const int t = threadIdx.x;
+    if (add_parked && t == 0) count[nb - 1] += (int)(start[nb] - start[nb - 1]);
+    __syncthreads();
     const int per = (nb + kScanBlock - 1) / kScanBlock;
     const int b0 = t * per, b1 = min(nb, b0 + per);
     int64_t c = 0;
@@ -93,12 +91,15 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(ParticleArrays src, Part
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kBlock * kScatterPer;
     int bq[kScatterPer], rq[kScatterPer];
+    float xq[kScatterPer], yq[kScatterPer];
 #pragma unroll
     for (int q = 0; q < kScatterPer; ++q) {
         const int64_t p = base + q * kBlock + threadIdx.x;
         bq[q] = -1;
         if (p < np) {
-            bq[q] = bin_of(src.x[p], src.y[p], n, tpr);
+            xq[q] = src.x[p];
+            yq[q] = src.y[p];
+            bq[q] = bin_of(xq[q], yq[q], n, tpr);
             rq[q] = atomicAdd(&lcnt[bq[q]], 1);
         }
     }
@@ -112,8 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(ParticleArrays src, Part
         const int64_t p = base + q * kBlock + threadIdx.x;
         if (bq[q] < 0) continue;
         const int64_t s = start[bq[q]] + lbase[bq[q]] + rq[q];
-        dst.x[s] = src.x[p];
-        dst.y[s] = src.y[p];
+        dst.x[s] = xq[q];
+        dst.y[s] = yq[q];
         dst.vx[s] = src.vx[p];
         dst.vy[s] = src.vy[p];
         dst.m[s] = src.m[p];
@@ -162,17 +163,18 @@ int bin_count_of(int n) {
     return tpr * tpr + 1;
 }
 
-hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, hipStream_t s) {
+hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, bool counted,
+                          hipStream_t s) {
     const int nb = bin_count_of(n);
-    hipError_t e = fill_zero(bs.count, sizeof(int) * (size_t)nb, s);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     if ((e = allow_lds(reinterpret_cast<const void*>(&k_bin_count), sizeof(int) * nb)) != hipSuccess) return e;
     if ((e = allow_lds(reinterpret_cast<const void*>(&k_bin_scatter), 2 * sizeof(int) * nb)) != hipSuccess) return e;
-    if (np > 0) {
+    if (!counted && (e = fill_zero(bs.count, sizeof(int) * (size_t)nb, s)) != hipSuccess) return e;
+    if (!counted && np > 0) {
         hipLaunchKernelGGL(k_bin_count, dim3(grid_for(np, 64, 2048)), dim3(kBlock), sizeof(int) * nb, s, cur.x, cur.y,
                            np, n, bs.count);
     }
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk, bs.start,
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk, counted ? 1 : 0, bs.start,
                        bs.chunk_start, bs.cursor);
     if (np > 0) {
         const int64_t blocks = (np + (int64_t)kBlock * kScatterPer - 1) / ((int64_t)kBlock * kScatterPer);

@@ -18,6 +18,14 @@ constexpr int kWin = kTile + 2 * kHalo + 1;          // 79: 49,928 B of int64 pe
 constexpr int kDepositBlock = 512;
 constexpr int kDepositChunkDefault = 65536;          // particles per deposit block (tunable)
 
+// Bin of a position: its 64x64 tile when in bounds (1 <= x, y < N-1, the
+// reference's rule, cuh:257, 497), else the parked bin tpr*tpr.
+__device__ __forceinline__ int bin_of(float x, float y, int n, int tpr) {
+    const float hi = (float)(n - 1);
+    if (!(x >= 1.0f && x < hi && y >= 1.0f && y < hi)) return tpr * tpr;
+    return ((int)y >> kTileLog) * tpr + ((int)x >> kTileLog);
+}
+
 struct ParticleArrays {
     float *x, *y, *vx, *vy, *m;
     uint32_t* id;   // load index of the particle in each slot
@@ -40,7 +48,10 @@ inline bool lattice_supported(int n) {
 // ----- binning (pm_binning.hip) -----------------------------------------------
 int bin_count_of(int n);   // in-bounds bins + 1 parked bin
 // Counting sort of cur into alt by bin; fills bs.  Caller swaps cur and alt.
-hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, hipStream_t s);
+// counted: bs.count already holds the bins of every particle outside the parked
+// range (filled by push_deposit), so the counting pass is skipped.
+hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, bool counted,
+                          hipStream_t s);
 hipError_t iota_ids(uint32_t* id, int64_t np, hipStream_t s);
 hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const float* a1, const float* a2,
                      const float* a3, float* o0, float* o1, float* o2, float* o3, hipStream_t s);
@@ -53,6 +64,12 @@ hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const floa
 int deposit_max_chunks(int n, int64_t capacity, int chunk);
 hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
                    float scale, int64_t* rho, unsigned long long* migrants, hipStream_t s);
+// Fused step tail: gather + kick-drift every particle of every in-bounds bin,
+// then deposit its new position into rho_next (zero on entry).  count_bins:
+// also histogram the new bins into bs.count (zero on entry) for the next sort.
+hipError_t push_deposit(int n, int mode, int wrap_mode, const float2* force, ParticleArrays p, BinState bs,
+                        int max_chunks, float dt, float scale, int64_t* rho_next, unsigned long long* migrants,
+                        bool count_bins, hipStream_t s);
 
 // ----- FFT field solve (pm_fft.hip) -------------------------------------------
 // Half spectrum layout: H[k * N + j], k in [0, N/2] (x frequency), j in [0, N)

@@ -208,6 +208,139 @@ hipError_t push_t(const float2* force, float* x, float* y, float* vx, float* vy,
     return hipGetLastError();
 }
 
+// Fused push + next deposit: one block per deposit chunk of one bin.  Each
+// particle is gathered and kick-drifted exactly as push_one does, written
+// back, and its new position deposited (int64, LDS window of the bin with
+// global-atomic spill) into the next step's mass lattice.  Optionally the new
+// bins are histogrammed for the next re-sort.
+template <int MODE, int WRAP, bool COUNT>
+__global__ __launch_bounds__(kDepositBlock) void k_push_deposit(const float2* __restrict__ force,
+                                                                float* __restrict__ x, float* __restrict__ y,
+                                                                float* __restrict__ vx, float* __restrict__ vy,
+                                                                const float* __restrict__ m, int n, float dt,
+                                                                const int64_t* __restrict__ start,
+                                                                const int* __restrict__ chunk_start, int chunk,
+                                                                float scale, unsigned long long* __restrict__ rho,
+                                                                unsigned long long* __restrict__ migrants,
+                                                                int* __restrict__ count) {
+    __shared__ unsigned long long win[kWin * kWin];
+    __shared__ unsigned int s_mig;
+    extern __shared__ int hist[];
+    const int tpr = n >> kTileLog, nin = tpr * tpr;
+    const int c = blockIdx.x;
+    const int total = chunk_start[nin];
+    if (c >= total) return;
+    int lo = 0, hi = nin;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (chunk_start[mid] <= c) lo = mid + 1; else hi = mid;
+    }
+    const int b = lo - 1;
+    const int64_t s0 = start[b] + (int64_t)(c - chunk_start[b]) * chunk;
+    const int64_t s1 = min(s0 + chunk, start[b + 1]);
+    const int wx0 = (b % tpr) * kTile - kHalo, wy0 = (b / tpr) * kTile - kHalo;
+    for (int e = threadIdx.x; e < kWin * kWin; e += kDepositBlock) win[e] = 0ull;
+    if (COUNT)
+        for (int e = threadIdx.x; e <= nin; e += kDepositBlock) hist[e] = 0;
+    if (threadIdx.x == 0) s_mig = 0;
+    __syncthreads();
+    const float fhi = (float)(n - 1);
+    unsigned int mig = 0;
+    constexpr int U = 4;
+    for (int64_t p0 = s0 + threadIdx.x; p0 < s1; p0 += U * kDepositBlock) {
+        float px[U], py[U], ux[U], uy[U], pm[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t p = p0 + u * kDepositBlock;
+            px[u] = -1.0f; py[u] = -1.0f; ux[u] = 0.0f; uy[u] = 0.0f; pm[u] = 0.0f;
+            if (p < s1) {
+                px[u] = __builtin_nontemporal_load(x + p);
+                py[u] = __builtin_nontemporal_load(y + p);
+                ux[u] = __builtin_nontemporal_load(vx + p);
+                uy[u] = __builtin_nontemporal_load(vy + p);
+                pm[u] = __builtin_nontemporal_load(m + p);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t p = p0 + u * kDepositBlock;
+            if (p >= s1) continue;
+            push_one<MODE, WRAP>(force, n, dt, px[u], py[u], ux[u], uy[u]);
+            __builtin_nontemporal_store(px[u], x + p);
+            __builtin_nontemporal_store(py[u], y + p);
+            __builtin_nontemporal_store(ux[u], vx + p);
+            __builtin_nontemporal_store(uy[u], vy + p);
+            if (COUNT) atomicAdd(&hist[bin_of(px[u], py[u], n, tpr)], 1);
+            if (!in_bounds(px[u], py[u], fhi)) continue;
+            const int xi = (int)px[u], yi = (int)py[u];
+            const int lx = xi - wx0, ly = yi - wy0;
+            const bool local = (unsigned)lx < (unsigned)(kWin - 1) && (unsigned)ly < (unsigned)(kWin - 1);
+            if (MODE == 0) {
+                const unsigned long long q = quantise(pm[u], scale);
+                if (local) atomicAdd(&win[ly * kWin + lx], q);
+                else { atomicAdd(rho + (size_t)yi * n + xi, q); ++mig; }
+            } else {
+                const float fx = px[u] - (float)xi, fy = py[u] - (float)yi;
+                const float ax = 1.0f - fx, ay = 1.0f - fy;
+                const unsigned long long q00 = quantise((ax * ay) * pm[u], scale);
+                const unsigned long long q10 = quantise((fx * ay) * pm[u], scale);
+                const unsigned long long q01 = quantise((ax * fy) * pm[u], scale);
+                const unsigned long long q11 = quantise((fx * fy) * pm[u], scale);
+                if (local) {
+                    unsigned long long* w = win + ly * kWin + lx;
+                    atomicAdd(w, q00);
+                    atomicAdd(w + 1, q10);
+                    atomicAdd(w + kWin, q01);
+                    atomicAdd(w + kWin + 1, q11);
+                } else {
+                    unsigned long long* g = rho + (size_t)yi * n + xi;
+                    atomicAdd(g, q00);
+                    atomicAdd(g + 1, q10);
+                    atomicAdd(g + n, q01);
+                    atomicAdd(g + n + 1, q11);
+                    ++mig;
+                }
+            }
+        }
+    }
+    if (mig) atomicAdd(&s_mig, mig);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kWin * kWin; e += kDepositBlock) {
+        const unsigned long long v = win[e];
+        if (v == 0ull) continue;
+        const int gx = wx0 + e % kWin, gy = wy0 + e / kWin;
+        atomicAdd(rho + (size_t)gy * n + gx, v);
+    }
+    if (COUNT)
+        for (int e = threadIdx.x; e <= nin; e += kDepositBlock)
+            if (hist[e]) atomicAdd(&count[e], hist[e]);
+    if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
+}
+
+template <int MODE, int WRAP>
+hipError_t push_deposit_t(int n, const float2* force, ParticleArrays p, BinState bs, int max_chunks, float dt,
+                          float scale, int64_t* rho_next, unsigned long long* migrants, bool count_bins,
+                          hipStream_t s) {
+    auto* r = reinterpret_cast<unsigned long long*>(rho_next);
+    const int nb = (n >> kTileLog) * (n >> kTileLog) + 1;
+    if (count_bins) {
+        const size_t lds = sizeof(int) * nb;
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit<MODE, WRAP, true>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL((k_push_deposit<MODE, WRAP, true>), dim3(max_chunks), dim3(kDepositBlock), lds, s, force,
+                           p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
+                           bs.count);
+    } else {
+        hipLaunchKernelGGL((k_push_deposit<MODE, WRAP, false>), dim3(max_chunks), dim3(kDepositBlock), 0, s, force,
+                           p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
+                           bs.count);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 int deposit_max_chunks(int n, int64_t capacity, int chunk) {
@@ -226,6 +359,18 @@ hipError_t deposit(int n, int mode, const float* x, const float* y, const float*
         hipLaunchKernelGGL((k_deposit_bin<1>), grid, dim3(kDepositBlock), 0, s, x, y, m, n, bs.start,
                            bs.chunk_start, bs.chunk, scale, r, migrants);
     return hipGetLastError();
+}
+
+hipError_t push_deposit(int n, int mode, int wrap_mode, const float2* force, ParticleArrays p, BinState bs,
+                        int max_chunks, float dt, float scale, int64_t* rho_next, unsigned long long* migrants,
+                        bool count_bins, hipStream_t s) {
+    if (mode == 0 && wrap_mode == 0)
+        return push_deposit_t<0, 0>(n, force, p, bs, max_chunks, dt, scale, rho_next, migrants, count_bins, s);
+    if (mode == 0)
+        return push_deposit_t<0, 1>(n, force, p, bs, max_chunks, dt, scale, rho_next, migrants, count_bins, s);
+    if (wrap_mode == 0)
+        return push_deposit_t<1, 0>(n, force, p, bs, max_chunks, dt, scale, rho_next, migrants, count_bins, s);
+    return push_deposit_t<1, 1>(n, force, p, bs, max_chunks, dt, scale, rho_next, migrants, count_bins, s);
 }
 
 hipError_t push(int n, int mode, const float2* force, float* x, float* y, float* vx, float* vy, int64_t np,

@@ -21,7 +21,7 @@ FORCE_SPECTRAL, FORCE_STENCIL = 0, 1
 FFT_FP32, FFT_BF16X3 = 0, 1
 LATTICE_MASS, LATTICE_POTENTIAL, LATTICE_FORCE, LATTICE_TRANSFER = range(4)
 STAGE_DEPOSIT, STAGE_FIELD, STAGE_PUSH = range(3)
-KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push", "sort"]
+KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push", "sort", "push_deposit"]
 
 
 class PMError(RuntimeError):

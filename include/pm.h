@@ -55,7 +55,7 @@ enum { PM_STAGE_DEPOSIT = 0, PM_STAGE_FIELD = 1, PM_STAGE_PUSH = 2 };
 /* timed kernels (pm_profile_read) */
 enum {
     PM_K_CLEAR = 0, PM_K_DEPOSIT = 1, PM_K_ROW_R2C = 2, PM_K_COL_CONV = 3, PM_K_ROW_C2R = 4,
-    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_SORT = 8, PM_K_COUNT = 9
+    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_SORT = 8, PM_K_PUSH_DEPOSIT = 9, PM_K_COUNT = 10
 };
 
 /* Replaces the compile-time namespace Constants (cuh:29-74, main.cu:5-25) and
@@ -74,7 +74,7 @@ typedef struct pm_config {
     int32_t mass_shift;    /* fixed-point bits (rho = q * 2^-shift); 0 = from the masses */
     int32_t device;        /* HIP device ordinal                                       */
     int32_t resort_interval; /* steps between re-binning the particles by lattice tile;
-                                0 = default (4).  Affects speed only, never results.    */
+                                0 = default (8).  Affects speed only, never results.    */
     int32_t reserved[6];
 } pm_config;
 
@@ -120,7 +120,9 @@ pm_status pm_load_verlet(pm_handle h, const float* x, const float* y, const floa
 
 /* Enqueue nsteps full steps on the handle's stream (asynchronous).  Replaces
  * the worker loop body (cuh:1336-1341): deposit, FFT convolution, gradient,
- * gather + update. */
+ * gather + update.  The deposit of each step after the first is fused into
+ * the previous step's gather + update; the mass lattice of the positions a
+ * step ends with is therefore already built when it returns. */
 pm_status pm_step(pm_handle h, int32_t nsteps);
 
 /* Enqueue one stage of a step (PM_STAGE_*), for stage-level checks. */

@@ -271,3 +271,42 @@ def test_tunables_do_not_change_results():
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [pm.NGP, pm.CIC])
+def test_fused_step_bit_exact(mode):
+    """pm_step fuses the next deposit into the push: after one step the
+    positions equal the oracle push on the step's force lattice, and the mass
+    lattice equals the oracle deposit of the new positions, bit for bit."""
+    n, npart = 512, 200000
+    x, y, vx, vy, m = ics.exponential_disk(npart, 256, 256, 25, 200, seed=31, masses="lognormal")
+    with make_sim(n, npart, mode, dt=0.05) as s:
+        s.load(x, y, vx, vy, m)
+        s.step(1)
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        got = s.read()
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        shift = s.info().mass_shift
+    want = orc.push32(n, mode, np.float32(0.05), 0, x, y, vx, vy, force)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint32), w.view(np.uint32))
+    assert np.array_equal(rho, orc.deposit(n, mode, shift, want[0], want[1], m))
+
+
+def test_fused_steps_equal_stage_sequence():
+    """Five fused steps (with re-sorts from the fused histogram) equal five
+    deposit/field/push stage sequences, bit for bit."""
+    n, npart = 512, 250000
+    x, y, vx, vy, m = ics.exponential_disk(npart, 256, 256, 30, 220, seed=32, masses="lognormal")
+    with make_sim(n, npart, pm.CIC, dt=0.05, resort_interval=2) as a, \
+            make_sim(n, npart, pm.CIC, dt=0.05, resort_interval=2) as b:
+        a.load(x, y, vx, vy, m)
+        b.load(x, y, vx, vy, m)
+        a.step(5)
+        for _ in range(5):
+            b.stage(pm.STAGE_DEPOSIT)
+            b.stage(pm.STAGE_FIELD)
+            b.stage(pm.STAGE_PUSH)
+        for ga, gb in zip(a.read(), b.read()):
+            assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+        assert a.info().sorts >= 3

@@ -240,7 +240,10 @@ def test_diagnostics_vs_numpy():
     assert np.isclose(d["px"], np.sum(m64 * gvx), rtol=1e-9, atol=1e-9)
     assert np.isclose(d["kinetic"], 0.5 * np.sum(m64 * (gvx.astype(np.float64) ** 2 + gvy.astype(np.float64) ** 2)),
                       rtol=1e-9)
-    assert np.isclose(d["potential"], 0.5 * np.sum(rho * 2.0 ** -shift * phi.astype(np.float64)), rtol=1e-9)
+    # potential energy pairs Phi with the lattice that produced it: the deposit of the loaded positions
+    rho0 = orc.deposit(n, pm.CIC, shift, x, y, m)
+    assert np.isclose(d["potential"], 0.5 * np.sum(rho0 * 2.0 ** -shift * phi.astype(np.float64)), rtol=1e-9)
+    assert np.array_equal(rho, orc.deposit(n, pm.CIC, shift, gx, gy, m))
     assert d["nonfinite"] == 0 and d["in_bounds"] == npart
 
 

@@ -583,11 +583,12 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
     if (k == "deposit_chunk") {
         if (value < pmk::kDepositBlock || value > (1 << 24) || (value % pmk::kDepositBlock) != 0)
             return fail(h, PM_EINVAL, "deposit_chunk must be a multiple of 512 in [512, 2^24]");
-        // chunk starts depend on the chunk size: re-bin on the next deposit
+        // chunk starts depend on the chunk size: re-bin now so every chunk-mapped
+        // kernel (deposit, fused push + deposit) sees a consistent map
         h->bins.chunk = (int)value;
         h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->cap, h->bins.chunk);
-        h->steps_since_sort = h->resort_interval;
-        return PM_OK;
+        PM_HIP(h, hipSetDevice(h->cfg.device));
+        return sort_now(h, false);
     }
     return fail(h, PM_EINVAL, "unknown parameter: " + k);
 }

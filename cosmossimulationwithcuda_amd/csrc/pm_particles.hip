@@ -208,22 +208,68 @@ hipError_t push_t(const float2* force, float* x, float* y, float* vx, float* vy,
     return hipGetLastError();
 }
 
-// Fused push + next deposit: one block per deposit chunk of one bin.  Each
-// particle is gathered and kick-drifted exactly as push_one does, written
-// back, and its new position deposited (int64, LDS window of the bin with
-// global-atomic spill) into the next step's mass lattice.  Optionally the new
-// bins are histogrammed for the next re-sort.
+// Gather + kick-drift of one particle with the force taken from the bin's
+// LDS window when its cell lies inside it, else from the lattice.  Same
+// arithmetic as push_one, so results are identical whichever path serves it.
+template <int MODE, int WRAP>
+__device__ __forceinline__ void push_one_win(const float2* __restrict__ fwin, int wx0, int wy0,
+                                             const float2* __restrict__ force, int n, float dt, float& px,
+                                             float& py, float& vx, float& vy) {
+    if (!in_bounds(px, py, (float)(n - 1))) return;
+    const int xi = (int)px, yi = (int)py;
+    const int lx = xi - wx0, ly = yi - wy0;
+    const bool local = (unsigned)lx < (unsigned)(kWin - 1) && (unsigned)ly < (unsigned)(kWin - 1);
+    const float2* f = local ? fwin + ly * kWin + lx : force + (size_t)yi * n + xi;
+    const int row = local ? kWin : n;
+    float ax, ay;
+    if (MODE == 0) {
+        const float2 f0 = f[0];
+        ax = f0.x;
+        ay = f0.y;
+    } else {
+        const float fx = px - (float)xi, fy = py - (float)yi;
+        const float wx = 1.0f - fx, wy = 1.0f - fy;
+        const float w00 = wx * wy, w10 = fx * wy, w01 = wx * fy, w11 = fx * fy;
+        const float2 f00 = f[0], f10 = f[1], f01 = f[row], f11 = f[row + 1];
+        ax = ((f00.x * w00 + f10.x * w10) + f01.x * w01) + f11.x * w11;
+        ay = ((f00.y * w00 + f10.y * w10) + f01.y * w01) + f11.y * w11;
+    }
+    const float span = (float)(n - 4);
+    float nvx = fmaf(ax, dt, vx);
+    float nvy = fmaf(ay, dt, vy);
+    float nx = fmaf(nvx, dt, px);
+    float ny = fmaf(nvy, dt, py);
+    if (nx < 2.0f) { nx += span; if (WRAP) nvx += span / dt; }
+    if (ny < 2.0f) { ny += span; if (WRAP) nvy += span / dt; }
+    if (nx > (float)(n - 2)) { nx -= span; if (WRAP) nvx -= span / dt; }
+    if (ny > (float)(n - 2)) { ny -= span; if (WRAP) nvy -= span / dt; }
+    px = nx;
+    py = ny;
+    vx = nvx;
+    vy = nvy;
+}
+
+// Fused push + next deposit: one block per deposit chunk of one bin.  The
+// bin's force window (tile + kHalo margin, float2) is staged in LDS so the
+// CIC gathers are LDS reads; each particle is kick-drifted, written back, and
+// its new position deposited (int64, LDS window with global-atomic spill)
+// into the next step's mass lattice.  Optionally the new bins are
+// histogrammed for the next re-sort.  Both windows take 99.9 KB of LDS: one
+// 1024-thread block per CU.
+constexpr int kFusedBlock = 1024;
+
 template <int MODE, int WRAP, bool COUNT>
-__global__ __launch_bounds__(kDepositBlock) void k_push_deposit(const float2* __restrict__ force,
-                                                                float* __restrict__ x, float* __restrict__ y,
-                                                                float* __restrict__ vx, float* __restrict__ vy,
-                                                                const float* __restrict__ m, int n, float dt,
-                                                                const int64_t* __restrict__ start,
-                                                                const int* __restrict__ chunk_start, int chunk,
-                                                                float scale, unsigned long long* __restrict__ rho,
-                                                                unsigned long long* __restrict__ migrants,
-                                                                int* __restrict__ count) {
+__global__ __launch_bounds__(kFusedBlock) void k_push_deposit(const float2* __restrict__ force,
+                                                              float* __restrict__ x, float* __restrict__ y,
+                                                              float* __restrict__ vx, float* __restrict__ vy,
+                                                              const float* __restrict__ m, int n, float dt,
+                                                              const int64_t* __restrict__ start,
+                                                              const int* __restrict__ chunk_start, int chunk,
+                                                              float scale, unsigned long long* __restrict__ rho,
+                                                              unsigned long long* __restrict__ migrants,
+                                                              int* __restrict__ count) {
     __shared__ unsigned long long win[kWin * kWin];
+    __shared__ float2 fwin[kWin * kWin];
     __shared__ unsigned int s_mig;
     extern __shared__ int hist[];
     const int tpr = n >> kTileLog, nin = tpr * tpr;
@@ -239,19 +285,24 @@ __global__ __launch_bounds__(kDepositBlock) void k_push_deposit(const float2* __
     const int64_t s0 = start[b] + (int64_t)(c - chunk_start[b]) * chunk;
     const int64_t s1 = min(s0 + chunk, start[b + 1]);
     const int wx0 = (b % tpr) * kTile - kHalo, wy0 = (b / tpr) * kTile - kHalo;
-    for (int e = threadIdx.x; e < kWin * kWin; e += kDepositBlock) win[e] = 0ull;
+    for (int e = threadIdx.x; e < kWin * kWin; e += kFusedBlock) {
+        win[e] = 0ull;
+        const int gx = wx0 + e % kWin, gy = wy0 + e / kWin;
+        fwin[e] = ((unsigned)gx < (unsigned)n && (unsigned)gy < (unsigned)n) ? force[(size_t)gy * n + gx]
+                                                                             : make_float2(0.f, 0.f);
+    }
     if (COUNT)
-        for (int e = threadIdx.x; e <= nin; e += kDepositBlock) hist[e] = 0;
+        for (int e = threadIdx.x; e <= nin; e += kFusedBlock) hist[e] = 0;
     if (threadIdx.x == 0) s_mig = 0;
     __syncthreads();
     const float fhi = (float)(n - 1);
     unsigned int mig = 0;
     constexpr int U = 4;
-    for (int64_t p0 = s0 + threadIdx.x; p0 < s1; p0 += U * kDepositBlock) {
+    for (int64_t p0 = s0 + threadIdx.x; p0 < s1; p0 += U * kFusedBlock) {
         float px[U], py[U], ux[U], uy[U], pm[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t p = p0 + u * kDepositBlock;
+            const int64_t p = p0 + u * kFusedBlock;
             px[u] = -1.0f; py[u] = -1.0f; ux[u] = 0.0f; uy[u] = 0.0f; pm[u] = 0.0f;
             if (p < s1) {
                 px[u] = __builtin_nontemporal_load(x + p);
@@ -262,14 +313,21 @@ __global__ __launch_bounds__(kDepositBlock) void k_push_deposit(const float2* __
             }
         }
 #pragma unroll
+        for (int u = 0; u < U; ++u)
+            push_one_win<MODE, WRAP>(fwin, wx0, wy0, force, n, dt, px[u], py[u], ux[u], uy[u]);
+#pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t p = p0 + u * kDepositBlock;
+            const int64_t p = p0 + u * kFusedBlock;
             if (p >= s1) continue;
-            push_one<MODE, WRAP>(force, n, dt, px[u], py[u], ux[u], uy[u]);
             __builtin_nontemporal_store(px[u], x + p);
             __builtin_nontemporal_store(py[u], y + p);
             __builtin_nontemporal_store(ux[u], vx + p);
             __builtin_nontemporal_store(uy[u], vy + p);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t p = p0 + u * kFusedBlock;
+            if (p >= s1) continue;
             if (COUNT) atomicAdd(&hist[bin_of(px[u], py[u], n, tpr)], 1);
             if (!in_bounds(px[u], py[u], fhi)) continue;
             const int xi = (int)px[u], yi = (int)py[u];
@@ -305,14 +363,14 @@ __global__ __launch_bounds__(kDepositBlock) void k_push_deposit(const float2* __
     }
     if (mig) atomicAdd(&s_mig, mig);
     __syncthreads();
-    for (int e = threadIdx.x; e < kWin * kWin; e += kDepositBlock) {
+    for (int e = threadIdx.x; e < kWin * kWin; e += kFusedBlock) {
         const unsigned long long v = win[e];
         if (v == 0ull) continue;
         const int gx = wx0 + e % kWin, gy = wy0 + e / kWin;
         atomicAdd(rho + (size_t)gy * n + gx, v);
     }
     if (COUNT)
-        for (int e = threadIdx.x; e <= nin; e += kDepositBlock)
+        for (int e = threadIdx.x; e <= nin; e += kFusedBlock)
             if (hist[e]) atomicAdd(&count[e], hist[e]);
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }
@@ -325,16 +383,16 @@ hipError_t push_deposit_t(int n, const float2* force, ParticleArrays p, BinState
     const int nb = (n >> kTileLog) * (n >> kTileLog) + 1;
     if (count_bins) {
         const size_t lds = sizeof(int) * nb;
-        if (lds > 65536) {
+        if (lds + 2 * sizeof(unsigned long long) * kWin * kWin > 65536) {
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit<MODE, WRAP, true>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL((k_push_deposit<MODE, WRAP, true>), dim3(max_chunks), dim3(kDepositBlock), lds, s, force,
+        hipLaunchKernelGGL((k_push_deposit<MODE, WRAP, true>), dim3(max_chunks), dim3(kFusedBlock), lds, s, force,
                            p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
                            bs.count);
     } else {
-        hipLaunchKernelGGL((k_push_deposit<MODE, WRAP, false>), dim3(max_chunks), dim3(kDepositBlock), 0, s, force,
+        hipLaunchKernelGGL((k_push_deposit<MODE, WRAP, false>), dim3(max_chunks), dim3(kFusedBlock), 0, s, force,
                            p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
                            bs.count);
     }

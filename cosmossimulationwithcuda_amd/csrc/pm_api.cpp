@@ -365,6 +365,7 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->alt.id = static_cast<uint32_t*>(p);
     h->nbins = pmk::bin_count_of(n);
     h->bins.chunk = pmk::kDepositChunkDefault;
+    h->bins.tl = pmk::tile_log_for(n);
     h->max_chunks = pmk::deposit_max_chunks(n, h->cap, h->bins.chunk);
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
     if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);

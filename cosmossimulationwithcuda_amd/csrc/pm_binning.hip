@@ -18,14 +18,14 @@ namespace pmk {
 namespace {
 
 __global__ __launch_bounds__(kBlock) void k_bin_count(const float* __restrict__ x, const float* __restrict__ y,
-                                                       int64_t np, int n, int* __restrict__ count) {
+                                                       int64_t np, int n, int tl, int* __restrict__ count) {
     extern __shared__ int hist[];
-    const int tpr = n >> kTileLog, nb = tpr * tpr + 1;
+    const int tpr = n >> tl, nb = tpr * tpr + 1;
     for (int i = threadIdx.x; i < nb; i += kBlock) hist[i] = 0;
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < np; p += stride)
-        atomicAdd(&hist[bin_of(x[p], y[p], n, tpr)], 1);
+        atomicAdd(&hist[bin_of(x[p], y[p], n, tl)], 1);
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += kBlock)
         if (hist[i]) atomicAdd(&count[i], hist[i]);
@@ -81,10 +81,10 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count
 // x, y, vx, vy, m, id to their sorted slots in the second buffer set.
 constexpr int kScatterPer = 8;
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(ParticleArrays src, ParticleArrays dst, int64_t np, int n,
-                                                         const int64_t* __restrict__ start,
+                                                         int tl, const int64_t* __restrict__ start,
                                                          int* __restrict__ cursor) {
     extern __shared__ int lds[];
-    const int tpr = n >> kTileLog, nb = tpr * tpr + 1;
+    const int tpr = n >> tl, nb = tpr * tpr + 1;
     int* lcnt = lds;
     int* lbase = lds + nb;
     for (int i = threadIdx.x; i < nb; i += kBlock) lcnt[i] = 0;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(ParticleArrays src, Part
         if (p < np) {
             xq[q] = src.x[p];
             yq[q] = src.y[p];
-            bq[q] = bin_of(xq[q], yq[q], n, tpr);
+            bq[q] = bin_of(xq[q], yq[q], n, tl);
             rq[q] = atomicAdd(&lcnt[bq[q]], 1);
         }
     }
@@ -159,7 +159,7 @@ int grid_for(int64_t np, int per_thread, int cap) {
 }  // namespace
 
 int bin_count_of(int n) {
-    const int tpr = n >> kTileLog;
+    const int tpr = n >> tile_log_for(n);
     return tpr * tpr + 1;
 }
 
@@ -172,14 +172,14 @@ hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t
     if (!counted && (e = fill_zero(bs.count, sizeof(int) * (size_t)nb, s)) != hipSuccess) return e;
     if (!counted && np > 0) {
         hipLaunchKernelGGL(k_bin_count, dim3(grid_for(np, 64, 2048)), dim3(kBlock), sizeof(int) * nb, s, cur.x, cur.y,
-                           np, n, bs.count);
+                           np, n, bs.tl, bs.count);
     }
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk, counted ? 1 : 0, bs.start,
                        bs.chunk_start, bs.cursor);
     if (np > 0) {
         const int64_t blocks = (np + (int64_t)kBlock * kScatterPer - 1) / ((int64_t)kBlock * kScatterPer);
         hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(kBlock), 2 * sizeof(int) * nb, s, cur, alt, np,
-                           n, bs.start, bs.cursor);
+                           n, bs.tl, bs.start, bs.cursor);
     }
     return hipGetLastError();
 }

@@ -8,22 +8,28 @@ namespace pmk {
 
 constexpr int kBlock = 256;   // threads per block for every streaming kernel (4 wave64s)
 
-// Particle binning: 64x64-cell tiles; a deposit block accumulates one chunk of
-// one tile into an LDS window of the tile plus a kHalo-cell margin (plus the
-// +1 column/row CIC reaches).
-constexpr int kTileLog = 6;
-constexpr int kTile = 1 << kTileLog;
+// Particle binning: square tiles of 2^tl cells; a particle kernel block works
+// on one chunk of one tile with LDS windows covering the tile plus a
+// kHalo-cell margin (plus the +1 column/row CIC reaches).  Tiles are 32 cells
+// for lattices up to 2048 (at most 4096 bins, 17.7 KB windows) and 64 above.
 constexpr int kHalo = 7;
-constexpr int kWin = kTile + 2 * kHalo + 1;          // 79: 49,928 B of int64 per window
 constexpr int kDepositBlock = 512;
-constexpr int kDepositChunkDefault = 65536;          // particles per deposit block (tunable)
+constexpr int kDepositChunkDefault = 32768;          // particles per particle-kernel block (tunable)
 
-// Bin of a position: its 64x64 tile when in bounds (1 <= x, y < N-1, the
-// reference's rule, cuh:257, 497), else the parked bin tpr*tpr.
-__device__ __forceinline__ int bin_of(float x, float y, int n, int tpr) {
+inline int tile_log_for(int n) { return n <= 2048 ? 5 : 6; }
+template <int TL>
+struct TileGeom {
+    static constexpr int kEdge = 1 << TL;
+    static constexpr int kWin = kEdge + 2 * kHalo + 1;
+};
+
+// Bin of a position: its tile when in bounds (1 <= x, y < N-1, the
+// reference's rule, cuh:257, 497), else the parked bin (N >> tl)^2.
+__device__ __forceinline__ int bin_of(float x, float y, int n, int tl) {
+    const int tpr = n >> tl;
     const float hi = (float)(n - 1);
     if (!(x >= 1.0f && x < hi && y >= 1.0f && y < hi)) return tpr * tpr;
-    return ((int)y >> kTileLog) * tpr + ((int)x >> kTileLog);
+    return ((int)y >> tl) * tpr + ((int)x >> tl);
 }
 
 struct ParticleArrays {
@@ -37,6 +43,7 @@ struct BinState {
     int* chunk_start;    // [nbins + 1] exclusive prefix of deposit chunks per in-bounds bin
     int* cursor;         // [nbins + 1] scatter cursors
     int chunk;           // particles per deposit block
+    int tl;              // log2 of the tile edge in cells
 };
 
 // Lattice sizes with compiled kernels (powers of two).

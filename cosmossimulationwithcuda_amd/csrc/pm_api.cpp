@@ -37,6 +37,7 @@ struct pm_sim {
     pmk::ParticleArrays cur{}, alt{};      // sorted state and the counting-sort target / read scratch
     pmk::BinState bins{};
     int nbins = 0;                         // in-bounds bins + parked bin
+    int nbins_alloc = 0;                   // bins the bin arrays were sized for
     int max_chunks = 0;
     int resort_interval = 8;
     int steps_since_sort = 0;
@@ -110,7 +111,7 @@ Layout layout_of(const pm_config* c) {
     const int64_t n = c->n_lattice, nk = n / 2 + 1, cap = std::max<int64_t>(c->n_particles, 1);
     Layout L{};
     L.particles = align256(4 * cap);
-    const int64_t nb = pmk::bin_count_of((int)n);
+    const int64_t nb = pmk::bin_count_of((int)n, pmk::tile_log_for((int)n));
     L.bins = align256(4 * (nb + 1)) * 3 + align256(8 * (nb + 2)) + 256;
     L.rho = align256(8 * n * n);
     L.H = align256(8 * nk * n);
@@ -363,10 +364,11 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     }
     if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->cur.id = static_cast<uint32_t*>(p);
     if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->alt.id = static_cast<uint32_t*>(p);
-    h->nbins = pmk::bin_count_of(n);
     h->bins.chunk = pmk::kDepositChunkDefault;
     h->bins.tl = pmk::tile_log_for(n);
-    h->max_chunks = pmk::deposit_max_chunks(n, h->cap, h->bins.chunk);
+    h->nbins = pmk::bin_count_of(n, h->bins.tl);
+    h->nbins_alloc = h->nbins;
+    h->max_chunks = pmk::deposit_max_chunks(n, h->bins.tl, h->cap, h->bins.chunk);
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
     if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
     {
@@ -587,7 +589,17 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         // chunk starts depend on the chunk size: re-bin now so every chunk-mapped
         // kernel (deposit, fused push + deposit) sees a consistent map
         h->bins.chunk = (int)value;
-        h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->cap, h->bins.chunk);
+        h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
+        PM_HIP(h, hipSetDevice(h->cfg.device));
+        return sort_now(h, false);
+    }
+    if (k == "tile_log") {
+        if (value != 5 && value != 6) return fail(h, PM_EINVAL, "tile_log must be 5 or 6");
+        if (pmk::bin_count_of(h->cfg.n_lattice, (int)value) > h->nbins_alloc)
+            return fail(h, PM_EINVAL, "tile_log too small for the bin arrays of this lattice");
+        h->bins.tl = (int)value;
+        h->nbins = pmk::bin_count_of(h->cfg.n_lattice, h->bins.tl);
+        h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
         PM_HIP(h, hipSetDevice(h->cfg.device));
         return sort_now(h, false);
     }

@@ -158,14 +158,14 @@ int grid_for(int64_t np, int per_thread, int cap) {
 
 }  // namespace
 
-int bin_count_of(int n) {
-    const int tpr = n >> tile_log_for(n);
+int bin_count_of(int n, int tl) {
+    const int tpr = n >> tl;
     return tpr * tpr + 1;
 }
 
 hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t np, BinState bs, bool counted,
                           hipStream_t s) {
-    const int nb = bin_count_of(n);
+    const int nb = bin_count_of(n, bs.tl);
     hipError_t e;
     if ((e = allow_lds(reinterpret_cast<const void*>(&k_bin_count), sizeof(int) * nb)) != hipSuccess) return e;
     if ((e = allow_lds(reinterpret_cast<const void*>(&k_bin_scatter), 2 * sizeof(int) * nb)) != hipSuccess) return e;

@@ -53,7 +53,7 @@ inline bool lattice_supported(int n) {
 }
 
 // ----- binning (pm_binning.hip) -----------------------------------------------
-int bin_count_of(int n);   // in-bounds bins + 1 parked bin
+int bin_count_of(int n, int tl);   // in-bounds bins + 1 parked bin
 // Counting sort of cur into alt by bin; fills bs.  Caller swaps cur and alt.
 // counted: bs.count already holds the bins of every particle outside the parked
 // range (filled by push_deposit), so the counting pass is skipped.
@@ -68,7 +68,7 @@ hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const floa
 // for the particles of every in-bounds bin (bs from the last sort_particles).
 // max_chunks: upper bound of the chunk count (grid size).  migrants counts the
 // contributions made outside the tile window.
-int deposit_max_chunks(int n, int64_t capacity, int chunk);
+int deposit_max_chunks(int n, int tl, int64_t capacity, int chunk);
 hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
                    float scale, int64_t* rho, unsigned long long* migrants, hipStream_t s);
 // Fused step tail: gather + kick-drift every particle of every in-bounds bin,

@@ -21,6 +21,8 @@
 namespace pmk {
 namespace {
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // (int)x in [1, N-2] for both axes (cuh:257, 497); NaN fails.
 __device__ __forceinline__ bool in_bounds(float x, float y, float hi) {
     return x >= 1.0f && x < hi && y >= 1.0f && y < hi;
@@ -227,62 +229,100 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     if (threadIdx.x == 0) s_mig = 0;
     __syncthreads();
     const float fhi = (float)(n - 1);
-    float* xb = x + s0;
-    float* yb = y + s0;
-    float* vxb = vx + s0;
-    float* vyb = vy + s0;
-    const float* mb = m + s0;
-    const int len = (int)(s1 - s0);
+    // Aligned groups of 4 slots: [q0, q1) covers [s0, s1); slots outside it
+    // belong to neighbouring chunks and are neither used nor written.  Reads of
+    // the partial groups stay inside the arrays (capacity is padded to 64).
+    const int64_t q0 = s0 >> 2, q1 = (s1 + 3) >> 2;
+    const int nq = (int)(q1 - q0);
+    v4f* X = reinterpret_cast<v4f*>(x) + q0;
+    v4f* Y = reinterpret_cast<v4f*>(y) + q0;
+    v4f* VX = reinterpret_cast<v4f*>(vx) + q0;
+    v4f* VY = reinterpret_cast<v4f*>(vy) + q0;
+    const v4f* M = reinterpret_cast<const v4f*>(m) + q0;
+    const int lo_k = (int)(s0 - 4 * q0);             // first valid lane of group 0
+    const int hi_k = (int)(s1 - 4 * (q1 - 1));       // valid lanes of the last group: [0, hi_k)
     unsigned int mig = 0;
-    constexpr int U = 4;
-    for (int i0 = threadIdx.x; i0 < len; i0 += U * BLK) {
-        float px[U], py[U], ux[U], uy[U], pm[U];
+    constexpr int U = 2;
+    for (int j0 = threadIdx.x; j0 < nq; j0 += U * BLK) {
+        float px[U][4], py[U][4], ux[U][4], uy[U][4], pm[U][4];
+        bool ok[U][4];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int i = i0 + u * BLK;
-            px[u] = -1.0f; py[u] = -1.0f; ux[u] = 0.0f; uy[u] = 0.0f; pm[u] = 0.0f;
-            if (i < len) {
-                px[u] = __builtin_nontemporal_load(xb + i);
-                py[u] = __builtin_nontemporal_load(yb + i);
-                ux[u] = __builtin_nontemporal_load(vxb + i);
-                uy[u] = __builtin_nontemporal_load(vyb + i);
-                pm[u] = __builtin_nontemporal_load(mb + i);
+            const int j = j0 + u * BLK;
+            v4f a = {-1.f, -1.f, -1.f, -1.f}, b = a, c = {0.f, 0.f, 0.f, 0.f}, d = c, e = c;
+            if (j < nq) {
+                a = __builtin_nontemporal_load(X + j);
+                b = __builtin_nontemporal_load(Y + j);
+                c = __builtin_nontemporal_load(VX + j);
+                d = __builtin_nontemporal_load(VY + j);
+                e = __builtin_nontemporal_load(M + j);
+            }
+            px[u][0] = a.x; px[u][1] = a.y; px[u][2] = a.z; px[u][3] = a.w;
+            py[u][0] = b.x; py[u][1] = b.y; py[u][2] = b.z; py[u][3] = b.w;
+            ux[u][0] = c.x; ux[u][1] = c.y; ux[u][2] = c.z; ux[u][3] = c.w;
+            uy[u][0] = d.x; uy[u][1] = d.y; uy[u][2] = d.z; uy[u][3] = d.w;
+            pm[u][0] = e.x; pm[u][1] = e.y; pm[u][2] = e.z; pm[u][3] = e.w;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                ok[u][k] = j < nq && (j > 0 || k >= lo_k) && (j < nq - 1 || k < hi_k);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!ok[u][k] || !in_bounds(px[u][k], py[u][k], fhi)) continue;
+                const int xi = (int)px[u][k], yi = (int)py[u][k];
+                const int lx = xi - wx0, ly = yi - wy0;
+                const float fx = px[u][k] - (float)xi, fy = py[u][k] - (float)yi;
+                float2 a;
+                if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
+                    a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
+                else
+                    a = gather<MODE>(force + (size_t)yi * n + xi, n, fx, fy);
+                kick_drift<WRAP>(a, n, dt, px[u][k], py[u][k], ux[u][k], uy[u][k]);
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (!in_bounds(px[u], py[u], fhi)) continue;
-            const int xi = (int)px[u], yi = (int)py[u];
-            const int lx = xi - wx0, ly = yi - wy0;
-            const float fx = px[u] - (float)xi, fy = py[u] - (float)yi;
-            float2 a;
-            if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
-                a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
-            else
-                a = gather<MODE>(force + (size_t)yi * n + xi, n, fx, fy);
-            kick_drift<WRAP>(a, n, dt, px[u], py[u], ux[u], uy[u]);
+            const int j = j0 + u * BLK;
+            if (j >= nq) continue;
+            if (ok[u][0] && ok[u][3]) {
+                __builtin_nontemporal_store(v4f{px[u][0], px[u][1], px[u][2], px[u][3]}, X + j);
+                __builtin_nontemporal_store(v4f{py[u][0], py[u][1], py[u][2], py[u][3]}, Y + j);
+                __builtin_nontemporal_store(v4f{ux[u][0], ux[u][1], ux[u][2], ux[u][3]}, VX + j);
+                __builtin_nontemporal_store(v4f{uy[u][0], uy[u][1], uy[u][2], uy[u][3]}, VY + j);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!ok[u][k]) continue;
+                    float* sx = reinterpret_cast<float*>(X + j) + k;
+                    float* sy = reinterpret_cast<float*>(Y + j) + k;
+                    float* svx = reinterpret_cast<float*>(VX + j) + k;
+                    float* svy = reinterpret_cast<float*>(VY + j) + k;
+                    *sx = px[u][k];
+                    *sy = py[u][k];
+                    *svx = ux[u][k];
+                    *svy = uy[u][k];
+                }
+            }
         }
+        bool spill[U][4];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int i = i0 + u * BLK;
-            if (i >= len) continue;
-            __builtin_nontemporal_store(px[u], xb + i);
-            __builtin_nontemporal_store(py[u], yb + i);
-            __builtin_nontemporal_store(ux[u], vxb + i);
-            __builtin_nontemporal_store(uy[u], vyb + i);
-        }
-        bool spill[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            spill[u] = false;
-            if (i0 + u * BLK >= len) continue;
-            if (COUNT) atomicAdd(&hist[bin_of(px[u], py[u], n, TL)], 1);
-            spill[u] = in_bounds(px[u], py[u], fhi) &&
-                       !deposit_local<MODE, W>(win, wx0, wy0, px[u], py[u], pm[u], scale);
+            for (int k = 0; k < 4; ++k) {
+                spill[u][k] = false;
+                if (!ok[u][k]) continue;
+                if (COUNT) atomicAdd(&hist[bin_of(px[u][k], py[u][k], n, TL)], 1);
+                spill[u][k] = in_bounds(px[u][k], py[u][k], fhi) &&
+                              !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale);
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (spill[u]) { deposit_global<MODE>(rho, n, px[u], py[u], pm[u], scale); ++mig; }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (spill[u][k]) { deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale); ++mig; }
     }
     if (mig) atomicAdd(&s_mig, mig);
     __syncthreads();
@@ -297,8 +337,6 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
 // Four consecutive particles per lane: 16-byte nontemporal loads and stores
 // of x, y, vx, vy; force from the global lattice.  Used by the single-stage
 // entry point (the stepping path uses the fused kernel).
-typedef float v4f __attribute__((ext_vector_type(4)));
-
 template <int MODE, int WRAP>
 __global__ __launch_bounds__(kBlock) void k_push4(const float2* __restrict__ force, v4f* __restrict__ x,
                                                    v4f* __restrict__ y, v4f* __restrict__ vx,
@@ -403,8 +441,7 @@ hipError_t push_t(const float2* force, float* x, float* y, float* vx, float* vy,
 
 }  // namespace
 
-int deposit_max_chunks(int n, int64_t capacity, int chunk) {
-    const int tl = tile_log_for(n);
+int deposit_max_chunks(int n, int tl, int64_t capacity, int chunk) {
     const int64_t nin = (int64_t)(n >> tl) * (n >> tl);
     return (int)((capacity + chunk - 1) / chunk + nin);
 }

@@ -149,8 +149,9 @@ pm_status pm_diagnostics(pm_handle h, double out[8]);
 pm_status pm_get_info(pm_handle h, pm_info* info);
 
 /* Runtime tunables that change speed, never results: "resort_interval"
- * (steps between particle re-binnings) and "deposit_chunk" (particles per
- * deposit block, a multiple of 512).  No counterpart in the reference, whose
+ * (steps between particle re-binnings), "deposit_chunk" (particles per
+ * particle-kernel block, a multiple of 512) and "tile_log" (bin edge 2^5 or
+ * 2^6 cells; not below the lattice's default).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
 

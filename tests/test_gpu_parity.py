@@ -262,15 +262,16 @@ def test_tunables_do_not_change_results():
     n, npart = 512, 300000
     x, y, vx, vy, m = ics.exponential_disk(npart, 256, 256, 30, 200, seed=21, masses="lognormal")
     outs = []
-    for chunk, resort in ((512, 1), (65536, 7), (4096, 3)):
+    for chunk, resort, tl in ((512, 1, 5), (65536, 7, 6), (4096, 3, 5), (32768, 2, 6)):
         with make_sim(n, npart, pm.CIC, dt=0.05) as s:
             s.load(x, y, vx, vy, m)
             s.set_param("deposit_chunk", chunk)
             s.set_param("resort_interval", resort)
+            s.set_param("tile_log", tl)
             s.step(6)
             outs.append(s.read())
             info = s.info()
-            print(f"chunk {chunk} resort {resort}: sorts {info.sorts} migrants {info.migrants}")
+            print(f"chunk {chunk} resort {resort} tile_log {tl}: sorts {info.sorts} migrants {info.migrants}")
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -76,49 +76,78 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count
     if (t == kScanBlock - 1) start[nb] = s_cnt[t];
 }
 
-// Counting-sort scatter: each block takes kScatterPer particles per thread,
-// claims per-bin ranges with one global atomic per (block, bin), and moves
-// x, y, vx, vy, m, id to their sorted slots in the second buffer set.
-constexpr int kScatterPer = 8;
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(ParticleArrays src, ParticleArrays dst, int64_t np, int n,
-                                                         int tl, const int64_t* __restrict__ start,
-                                                         int* __restrict__ cursor) {
+// Counting-sort scatter.  A block of kScatBlock threads takes kScatGroups
+// aligned groups of 4 particles per thread (16-byte source reads), ranks each
+// particle within its bin in LDS, claims per-bin ranges with one global
+// atomic per (block, bin), and moves x, y, vx, vy, m, id to their sorted
+// slots in the second buffer set.  Lanes of one LDS-atomic instruction that
+// share a bin receive consecutive ranks, so their writes are contiguous.
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+constexpr int kScatBlock = 1024;
+constexpr int kScatGroups = 2;
+constexpr int kScatPerBlock = kScatBlock * kScatGroups * 4;
+
+__global__ __launch_bounds__(kScatBlock) void k_bin_scatter(ParticleArrays src, ParticleArrays dst, int64_t np, int n,
+                                                             int tl, const int64_t* __restrict__ start,
+                                                             int* __restrict__ cursor) {
     extern __shared__ int lds[];
     const int tpr = n >> tl, nb = tpr * tpr + 1;
     int* lcnt = lds;
     int* lbase = lds + nb;
-    for (int i = threadIdx.x; i < nb; i += kBlock) lcnt[i] = 0;
+    for (int i = threadIdx.x; i < nb; i += kScatBlock) lcnt[i] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kBlock * kScatterPer;
-    int bq[kScatterPer], rq[kScatterPer];
-    float xq[kScatterPer], yq[kScatterPer];
+    const int64_t g0 = (int64_t)blockIdx.x * kScatBlock * kScatGroups;
+    int bq[kScatGroups][4], rq[kScatGroups][4];
+    float xq[kScatGroups][4], yq[kScatGroups][4];
 #pragma unroll
-    for (int q = 0; q < kScatterPer; ++q) {
-        const int64_t p = base + q * kBlock + threadIdx.x;
-        bq[q] = -1;
-        if (p < np) {
-            xq[q] = src.x[p];
-            yq[q] = src.y[p];
-            bq[q] = bin_of(xq[q], yq[q], n, tl);
-            rq[q] = atomicAdd(&lcnt[bq[q]], 1);
+    for (int g = 0; g < kScatGroups; ++g) {
+        const int64_t q = g0 + g * kScatBlock + threadIdx.x;
+        v4f X = {0.f, 0.f, 0.f, 0.f}, Y = X;
+        if (4 * q < np) {   // the arrays are padded to 64 slots, so the last group reads in bounds
+            X = reinterpret_cast<const v4f*>(src.x)[q];
+            Y = reinterpret_cast<const v4f*>(src.y)[q];
+        }
+        xq[g][0] = X.x; xq[g][1] = X.y; xq[g][2] = X.z; xq[g][3] = X.w;
+        yq[g][0] = Y.x; yq[g][1] = Y.y; yq[g][2] = Y.z; yq[g][3] = Y.w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bq[g][k] = -1;
+            if (4 * q + k < np) {
+                bq[g][k] = bin_of(xq[g][k], yq[g][k], n, tl);
+                rq[g][k] = atomicAdd(&lcnt[bq[g][k]], 1);
+            }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kScatterPer; ++q)
-        if (bq[q] >= 0 && rq[q] == 0) lbase[bq[q]] = atomicAdd(&cursor[bq[q]], lcnt[bq[q]]);
+    for (int g = 0; g < kScatGroups; ++g)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (bq[g][k] >= 0 && rq[g][k] == 0) lbase[bq[g][k]] = atomicAdd(&cursor[bq[g][k]], lcnt[bq[g][k]]);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kScatterPer; ++q) {
-        const int64_t p = base + q * kBlock + threadIdx.x;
-        if (bq[q] < 0) continue;
-        const int64_t s = start[bq[q]] + lbase[bq[q]] + rq[q];
-        dst.x[s] = xq[q];
-        dst.y[s] = yq[q];
-        dst.vx[s] = src.vx[p];
-        dst.vy[s] = src.vy[p];
-        dst.m[s] = src.m[p];
-        dst.id[s] = src.id[p];
+    for (int g = 0; g < kScatGroups; ++g) {
+        const int64_t q = g0 + g * kScatBlock + threadIdx.x;
+        if (4 * q >= np) continue;
+        const v4f VX = reinterpret_cast<const v4f*>(src.vx)[q];
+        const v4f VY = reinterpret_cast<const v4f*>(src.vy)[q];
+        const v4f M = reinterpret_cast<const v4f*>(src.m)[q];
+        const v4u I = reinterpret_cast<const v4u*>(src.id)[q];
+        const float vxq[4] = {VX.x, VX.y, VX.z, VX.w}, vyq[4] = {VY.x, VY.y, VY.z, VY.w};
+        const float mq[4] = {M.x, M.y, M.z, M.w};
+        const unsigned iq[4] = {I.x, I.y, I.z, I.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (bq[g][k] < 0) continue;
+            const int64_t s = start[bq[g][k]] + lbase[bq[g][k]] + rq[g][k];
+            dst.x[s] = xq[g][k];
+            dst.y[s] = yq[g][k];
+            dst.vx[s] = vxq[k];
+            dst.vy[s] = vyq[k];
+            dst.m[s] = mq[k];
+            dst.id[s] = iq[k];
+        }
     }
 }
 
@@ -177,8 +206,8 @@ hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk, counted ? 1 : 0, bs.start,
                        bs.chunk_start, bs.cursor);
     if (np > 0) {
-        const int64_t blocks = (np + (int64_t)kBlock * kScatterPer - 1) / ((int64_t)kBlock * kScatterPer);
-        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(kBlock), 2 * sizeof(int) * nb, s, cur, alt, np,
+        const int64_t blocks = (np + kScatPerBlock - 1) / kScatPerBlock;
+        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(kScatBlock), 2 * sizeof(int) * nb, s, cur, alt, np,
                            n, bs.tl, bs.start, bs.cursor);
     }
     return hipGetLastError();

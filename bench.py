@@ -87,6 +87,21 @@ def cpu_baseline(n, mode, dt, arrs, steps, threads):
     return len(x) * steps / el, el, orc.num_threads()
 
 
+def copy_probe(nbytes):
+    """One device-to-device hipMemcpy of nbytes (a copy kernel of known size)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 0, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"# copy probe: {nbytes} bytes device to device", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +111,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for cpu_baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the run, copy 1 GiB device-to-device (hipMemcpy) as the calibration "
+                         "kernel of known size for rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -172,6 +190,8 @@ def main():
         "diagnostics": {k: diag[k] for k in ("mass", "kinetic", "potential", "nonfinite", "in_bounds")},
     }
     del sim
+    if args.copy_probe:
+        copy_probe(1 << 30)
 
     if args.cpu_steps > 0:
         threads = args.cpu_threads or os.cpu_count()

@@ -14,7 +14,7 @@ constexpr int kBlock = 256;   // threads per block for every streaming kernel (4
 // for lattices up to 2048 (at most 4096 bins, 17.7 KB windows) and 64 above.
 constexpr int kHalo = 7;
 constexpr int kDepositBlock = 512;
-constexpr int kDepositChunkDefault = 32768;          // particles per particle-kernel block (tunable)
+constexpr int kDepositChunkDefault = 16384;          // particles per particle-kernel block (tunable)
 
 inline int tile_log_for(int n) { return n <= 2048 ? 5 : 6; }
 template <int TL>

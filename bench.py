@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="colliding_disks_2048_100M_cic", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=30, help="timed oracle steps for cpu_baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",

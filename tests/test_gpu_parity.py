@@ -314,3 +314,44 @@ def test_fused_steps_equal_stage_sequence():
         for ga, gb in zip(a.read(), b.read()):
             assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
         assert a.info().sorts >= 3
+
+
+def test_config2_full_size():
+    """Config 2 at full size: 2048^2 lattice, 20M exponential-disk particles
+    (r_d 128, truncated at 900, lognormal masses), CIC, dt 0.005.  One fused
+    step is checked bit for bit (deposit and push mirrors on the GPU force
+    lattice), the field against FP64 (1e-5), then 20 more steps against
+    size-independent invariants: mass and in-bounds count conserved, momentum
+    drift small against the momentum scale, state finite."""
+    n, npart, dt = 2048, 20_000_000, 0.005
+    x, y, vx, vy, m = ics.single_disk(npart, seed=2)
+    with make_sim(n, npart, pm.CIC, dt=dt) as s:
+        s.load(x, y, vx, vy, m)
+        shift = s.info().mass_shift
+        s.stage(pm.STAGE_DEPOSIT)
+        rho0 = s.read_lattice(pm.LATTICE_MASS)
+        assert np.array_equal(rho0, orc.deposit(n, pm.CIC, shift, x, y, m))
+        d0 = s.diagnostics()
+        s.step(1)
+        phi = s.read_lattice(pm.LATTICE_POTENTIAL)
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        got = s.read()
+        rho1 = s.read_lattice(pm.LATTICE_MASS)
+        want = orc.push32(n, pm.CIC, np.float32(dt), 0, x, y, vx, vy, force)
+        for g, w in zip(got, want):
+            assert np.array_equal(g.view(np.uint32), w.view(np.uint32))
+        assert np.array_equal(rho1, orc.deposit(n, pm.CIC, shift, want[0], want[1], m))
+        phi64 = orc.potential(n, orc.transfer(n, pm.CIC, 1.0, 1e-3), shift, rho0)
+        fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+        e_phi = rel_l2(phi, phi64)
+        e_f = float(np.sqrt(np.sum((force[..., 0] - fx64) ** 2 + (force[..., 1] - fy64) ** 2)
+                            / np.sum(fx64 ** 2 + fy64 ** 2)))
+        print(f"config 2: potential rel L2 {e_phi:.3e}, force rel L2 {e_f:.3e}")
+        assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD
+        s.step(20)
+        d = s.diagnostics()
+    pscale = float(np.sum(m.astype(np.float64) * np.hypot(vx, vy)))
+    print(f"config 2 after 21 steps: mass {d['mass']:.6e}, px {d['px']:.3e}, py {d['py']:.3e} "
+          f"(momentum scale {pscale:.3e}), KE {d['kinetic']:.4e}, PE {d['potential']:.4e}")
+    assert d["mass"] == d0["mass"] and d["in_bounds"] == npart and d["nonfinite"] == 0
+    assert abs(d["px"] - d0["px"]) < 1e-3 * pscale and abs(d["py"] - d0["py"]) < 1e-3 * pscale

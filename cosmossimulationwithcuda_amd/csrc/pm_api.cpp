@@ -43,9 +43,11 @@ struct pm_sim {
     int steps_since_sort = 0;
     int64_t sorts = 0;
     unsigned long long* migrants = nullptr;
-    int64_t* rho = nullptr;                // mass lattice of the current positions (when rho_valid)
-    int64_t* rho_next = nullptr;           // target of the fused push + deposit
-    const int64_t* rho_field = nullptr;    // lattice the last field solve consumed (potential energy)
+    int64_t* rho_buf[2] = {nullptr, nullptr};
+    int rho_cur = 0;                       // rho_buf[rho_cur]: mass lattice of the current positions (when rho_valid)
+    int field_parity = -1;                 // rho_buf index the last field solve consumed (potential energy)
+    pmk::FieldDesc* fdesc = nullptr;       // device descriptors, [p] reads rho_buf[p]
+    pmk::FftPlan plan{};
     bool rho_valid = false;
     float2 *H = nullptr, *Hfy = nullptr;
     float* T = nullptr;
@@ -183,6 +185,25 @@ pm_status resolve_profile(pm_sim* h) {
     return PM_OK;
 }
 
+// Device descriptors of the field passes, one per mass-lattice parity.
+pm_status upload_field_desc(pm_sim* h) {
+    pmk::FieldDesc d[2];
+    for (int p = 0; p < 2; ++p) {
+        d[p].rho = h->rho_buf[p];
+        d[p].H = h->H;
+        d[p].Hfy = h->Hfy;
+        d[p].T = h->T;
+        d[p].dk = h->dk;
+        d[p].phi = h->phi;
+        d[p].force = h->force;
+        d[p].inv_scale = std::ldexp(1.0f, -h->shift);
+        d[p].gravity = h->cfg.gravity;
+    }
+    PM_HIP(h, hipMemcpyAsync(h->fdesc, d, sizeof(d), hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
 // ---- stages -----------------------------------------------------------------
 pm_status sort_now(pm_sim* h, bool counted) {
     KernelTimer kt(h, PM_K_SORT);
@@ -201,12 +222,12 @@ pm_status stage_deposit(pm_sim* h) {
     }
     {
         KernelTimer kt(h, PM_K_CLEAR);
-        PM_HIP(h, pmk::fill_zero(h->rho, (size_t)8 * n * n, h->stream));
+        PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur], (size_t)8 * n * n, h->stream));
     }
     {
         KernelTimer kt(h, PM_K_DEPOSIT);
         PM_HIP(h, pmk::deposit(n, h->cfg.mode, h->cur.x, h->cur.y, h->cur.m, h->bins, h->max_chunks,
-                               std::ldexp(1.0f, h->shift), h->rho, h->migrants, h->stream));
+                               std::ldexp(1.0f, h->shift), h->rho_buf[h->rho_cur], h->migrants, h->stream));
     }
     h->rho_valid = true;
     return PM_OK;
@@ -215,25 +236,26 @@ pm_status stage_deposit(pm_sim* h) {
 pm_status stage_field(pm_sim* h) {
     const int n = h->cfg.n_lattice;
     const bool spectral = h->cfg.force_mode == PM_FORCE_SPECTRAL;
+    const pmk::FieldDesc* d = h->fdesc + h->rho_cur;
     {
         KernelTimer kt(h, PM_K_ROW_R2C);
-        PM_HIP(h, pmk::row_r2c(n, h->rho, std::ldexp(1.0f, -h->shift), h->tw, h->H, h->stream));
-        h->rho_field = h->rho;
+        PM_HIP(h, pmk::row_r2c(n, d, 1, h->tw, h->plan, h->stream));
+        h->field_parity = h->rho_cur;
     }
     {
         KernelTimer kt(h, PM_K_COL_CONV);
-        PM_HIP(h, pmk::col_conv(n, h->H, h->Hfy, h->T, h->dk, h->tw, spectral, h->stream));
+        PM_HIP(h, pmk::col_conv(n, d, 1, h->tw, h->plan, spectral, h->stream));
     }
     {
         KernelTimer kt(h, PM_K_ROW_C2R);
-        PM_HIP(h, pmk::row_c2r(n, h->H, h->Hfy, h->dk, h->tw, h->phi, h->force, spectral, h->stream));
+        PM_HIP(h, pmk::row_c2r(n, d, 1, h->tw, h->plan, spectral, h->stream));
     }
     if (spectral) {
         KernelTimer kt(h, PM_K_EDGE);
-        PM_HIP(h, pmk::edge_force(n, h->phi, h->cfg.gravity, h->force, h->stream));
+        PM_HIP(h, pmk::edge_force(n, d, 1, h->stream));
     } else {
         KernelTimer kt(h, PM_K_STENCIL);
-        PM_HIP(h, pmk::stencil_force(n, h->phi, h->cfg.gravity, h->force, h->stream));
+        PM_HIP(h, pmk::stencil_force(n, d, 1, h->stream));
     }
     return PM_OK;
 }
@@ -248,22 +270,22 @@ pm_status stage_push(pm_sim* h) {
 }
 
 // Step tail with the next deposit fused into the push; the mass lattice of
-// the new positions is left in h->rho for the next field solve.
+// the new positions is left in rho_buf[rho_cur] for the next field solve.
 pm_status stage_push_deposit(pm_sim* h) {
     const int n = h->cfg.n_lattice;
     const bool due = h->steps_since_sort + 1 >= h->resort_interval;
     {
         KernelTimer kt(h, PM_K_CLEAR);
-        PM_HIP(h, pmk::fill_zero(h->rho_next, (size_t)8 * n * n, h->stream));
+        PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur ^ 1], (size_t)8 * n * n, h->stream));
         if (due) PM_HIP(h, pmk::fill_zero(h->bins.count, sizeof(int) * (size_t)h->nbins, h->stream));
     }
     {
         KernelTimer kt(h, PM_K_PUSH_DEPOSIT);
         PM_HIP(h, pmk::push_deposit(n, h->cfg.mode, h->cfg.wrap_mode, h->force, h->cur, h->bins, h->max_chunks,
-                                    h->cfg.dt, std::ldexp(1.0f, h->shift), h->rho_next, h->migrants, due,
-                                    h->stream));
+                                    h->cfg.dt, std::ldexp(1.0f, h->shift), h->rho_buf[h->rho_cur ^ 1], h->migrants,
+                                    due, h->stream));
     }
-    std::swap(h->rho, h->rho_next);
+    h->rho_cur ^= 1;
     h->rho_valid = true;
     h->steps_since_sort += 1;
     if (due) return sort_now(h, true);
@@ -290,14 +312,13 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
         PM_HIP(h, hipMemcpyAsync(h->cur.m, m, b, hipMemcpyHostToDevice, h->stream));
     }
     h->np = n;
+    h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
     PM_HIP(h, pmk::iota_ids(h->cur.id, n, h->stream));
     h->rho_valid = false;
-    h->rho_field = nullptr;
+    h->field_parity = -1;
     pm_status st = sort_now(h, false);
     if (st != PM_OK) return st;
-    PM_HIP(h, hipStreamSynchronize(h->stream));
-    h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
-    return PM_OK;
+    return upload_field_desc(h);   // inv_scale follows the shift
 }
 
 }  // namespace
@@ -380,8 +401,11 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
         h->bins.start = reinterpret_cast<int64_t*>(bp + 3 * ib);
         h->migrants = reinterpret_cast<unsigned long long*>(bp + 3 * ib + align256(8 * (int64_t)(h->nbins + 2)));
     }
-    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho = static_cast<int64_t*>(p);
-    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_next = static_cast<int64_t*>(p);
+    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_buf[0] = static_cast<int64_t*>(p);
+    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_buf[1] = static_cast<int64_t*>(p);
+    if ((st = dalloc(h, &p, align256(2 * sizeof(pmk::FieldDesc)))) != PM_OK) return bail(st);
+    h->fdesc = static_cast<pmk::FieldDesc*>(p);
+    h->plan = pmk::default_plan(n);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->Hfy = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.T)) != PM_OK) return bail(st); h->T = static_cast<float*>(p);
@@ -418,11 +442,12 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     // deterministic initial lattices (force/potential read before any step are zero)
     if (pmk::fill_zero(h->phi, (size_t)4 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||
-        pmk::fill_zero(h->rho, (size_t)8 * n * n, h->stream) != hipSuccess ||
+        pmk::fill_zero(h->rho_buf[0], (size_t)8 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->H, (size_t)8 * nk * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->bins.count, (size_t)L.bins, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(fail(h, PM_EDEVICE, "lattice clear failed"));
+    if ((st = upload_field_desc(h)) != PM_OK) return bail(st);
     *out = h;
     return PM_OK;
 }
@@ -529,7 +554,7 @@ pm_status pm_read_lattice(pm_handle h, int32_t which, void* dst, size_t bytes) {
     const void* src = nullptr;
     size_t need = 0;
     switch (which) {
-        case PM_LATTICE_MASS: src = h->rho; need = 8 * n * n; break;
+        case PM_LATTICE_MASS: src = h->rho_buf[h->rho_cur]; need = 8 * n * n; break;
         case PM_LATTICE_POTENTIAL: src = h->phi; need = 4 * n * n; break;
         case PM_LATTICE_FORCE: src = h->force; need = 8 * n * n; break;
         case PM_LATTICE_TRANSFER: src = h->T; need = 4 * nk * n; break;
@@ -564,8 +589,9 @@ pm_status pm_diagnostics(pm_handle h, double out[8]) {
     const int lb = pmk::diag_lattice_blocks(n);
     std::vector<double> lp((size_t)lb, 0.0);
     double pe = 0.0;
-    if (h->rho_field) {
-        PM_HIP(h, pmk::diag_lattice(n, h->rho_field, std::ldexp(1.0, -h->shift), h->phi, h->diag, h->stream));
+    if (h->field_parity >= 0) {
+        PM_HIP(h, pmk::diag_lattice(n, h->rho_buf[h->field_parity], std::ldexp(1.0, -h->shift), h->phi, h->diag,
+                                    h->stream));
         PM_HIP(h, hipMemcpyAsync(lp.data(), h->diag, lp.size() * 8, hipMemcpyDeviceToHost, h->stream));
         PM_HIP(h, hipStreamSynchronize(h->stream));
         for (double v : lp) pe += v;
@@ -603,8 +629,160 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         PM_HIP(h, hipSetDevice(h->cfg.device));
         return sort_now(h, false);
     }
+    if (k == "fft_rp" || k == "fft_cb" || k == "fft_radix") {
+        pmk::FftPlan p = h->plan;
+        (k == "fft_rp" ? p.rp : k == "fft_cb" ? p.cb : p.radix) = (int)value;
+        if (!pmk::plan_valid(h->cfg.n_lattice, p)) return fail(h, PM_EINVAL, "FFT plan not available for this lattice");
+        h->plan = p;
+        return PM_OK;
+    }
     return fail(h, PM_EINVAL, "unknown parameter: " + k);
 }
+
+pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
+    if (!h || !name || !value) return fail(h, PM_EINVAL, "null argument");
+    const std::string k(name);
+    if (k == "resort_interval") *value = h->resort_interval;
+    else if (k == "deposit_chunk") *value = h->bins.chunk;
+    else if (k == "tile_log") *value = h->bins.tl;
+    else if (k == "fft_rp") *value = h->plan.rp;
+    else if (k == "fft_cb") *value = h->plan.cb;
+    else if (k == "fft_radix") *value = h->plan.radix;
+    else return fail(h, PM_EINVAL, "unknown parameter: " + k);
+    return PM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device time of `reps` calls of fn on the handle's stream (HIP events).
+template <typename F>
+pm_status time_on_stream(pm_sim* h, int reps, F fn, float* ms) {
+    hipEvent_t a = nullptr, b = nullptr;
+    PM_HIP(h, hipEventCreate(&a));
+    PM_HIP(h, hipEventCreate(&b));
+    pm_status st = PM_OK;
+    if ((st = fn()) == PM_OK) {                        // warm-up (first-touch, code load)
+        (void)hipEventRecord(a, h->stream);
+        for (int r = 0; r < reps && st == PM_OK; ++r) st = fn();
+        (void)hipEventRecord(b, h->stream);
+        if (st == PM_OK && hipEventSynchronize(b) == hipSuccess) (void)hipEventElapsedTime(ms, a, b);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return st;
+}
+
+}  // namespace
+
+pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (reps <= 0) reps = 5;
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    const bool prof = h->profiling;
+    h->profiling = false;
+    pm_status st = PM_OK;
+    const int n = h->cfg.n_lattice;
+    if (flags & PM_TUNE_FFT) {
+        // Field solves only read the mass lattice: repeating them changes no state.
+        if (!h->rho_valid && (st = stage_deposit(h)) != PM_OK) { h->profiling = prof; return st; }
+        pmk::FftPlan best = h->plan;
+        float best_ms = 1e30f;
+        for (int radix : {4, 8})
+            for (int rp : {1, 2, 4})
+                for (int cb : {1, 2, 4, 8}) {
+                    pmk::FftPlan p;
+                    p.rp = rp; p.cb = cb; p.radix = radix;
+                    if (!pmk::plan_valid(n, p)) continue;
+                    h->plan = p;
+                    float ms = 0.f;
+                    if ((st = time_on_stream(h, reps, [&] { return stage_field(h); }, &ms)) != PM_OK) break;
+                    if (ms < best_ms) { best_ms = ms; best = p; }
+                }
+        h->plan = best;
+        if (st != PM_OK) { h->profiling = prof; return st; }
+        if ((st = stage_field(h)) != PM_OK) { h->profiling = prof; return st; }
+    }
+    if ((flags & PM_TUNE_PARTICLES) && h->np > 0) {
+        // Particle trials advance the state: snapshot it and restore it after.
+        const size_t pb = (size_t)h->np * 4, lb = (size_t)8 * n * n;
+        char* snap = nullptr;
+        hipError_t e = hipMalloc(&snap, 6 * pb + 2 * lb);
+        if (e != hipSuccess) { h->profiling = prof; return hip_fail(h, e, "hipMalloc(autotune snapshot)"); }
+        void* src[6] = {h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, h->cur.m, h->cur.id};
+        for (int a = 0; a < 6; ++a)
+            (void)hipMemcpyAsync(snap + a * pb, src[a], pb, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(snap + 6 * pb, h->rho_buf[0], lb, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(snap + 6 * pb + lb, h->rho_buf[1], lb, hipMemcpyDeviceToDevice, h->stream);
+        const pmk::ParticleArrays cur0 = h->cur, alt0 = h->alt;
+        const int rho_cur0 = h->rho_cur, since0 = h->steps_since_sort, fpar0 = h->field_parity;
+        const bool valid0 = h->rho_valid;
+        const int64_t sorts0 = h->sorts;
+        const int chunk0 = h->bins.chunk, tl0 = h->bins.tl, resort0 = h->resort_interval;
+        int best_chunk = chunk0, best_tl = tl0, best_resort = resort0;
+        float best_rate = 1e30f;
+        const int tl_default = pmk::tile_log_for(n);
+        for (int tl : {5, 6}) {
+            if (tl < tl_default) continue;
+            for (int chunk : {8192, 16384, 32768}) {
+                for (int resort : {4, 8, 12}) {
+                    h->bins.tl = tl;
+                    h->nbins = pmk::bin_count_of(n, tl);
+                    h->bins.chunk = chunk;
+                    h->max_chunks = pmk::deposit_max_chunks(n, tl, h->cap, chunk);
+                    h->resort_interval = resort;
+                    if ((st = sort_now(h, false)) != PM_OK) break;
+                    if (!h->rho_valid && (st = stage_deposit(h)) != PM_OK) break;
+                    float ms = 0.f;
+                    st = time_on_stream(h, resort, [&] {
+                        pm_status s1 = stage_field(h);
+                        return s1 != PM_OK ? s1 : stage_push_deposit(h);
+                    }, &ms);
+                    if (st != PM_OK) break;
+                    const float per_step = ms / (float)resort;
+                    if (per_step < best_rate) { best_rate = per_step; best_chunk = chunk; best_tl = tl; best_resort = resort; }
+                }
+                if (st != PM_OK) break;
+            }
+            if (st != PM_OK) break;
+        }
+        // restore the state exactly as it was
+        h->cur = cur0;
+        h->alt = alt0;
+        void* dst[6] = {h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, h->cur.m, h->cur.id};
+        for (int a = 0; a < 6; ++a)
+            (void)hipMemcpyAsync(dst[a], snap + a * pb, pb, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(h->rho_buf[0], snap + 6 * pb, lb, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(h->rho_buf[1], snap + 6 * pb + lb, lb, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(snap);
+        h->rho_cur = rho_cur0;
+        h->steps_since_sort = since0;
+        h->field_parity = fpar0;
+        h->rho_valid = valid0;
+        h->sorts = sorts0;
+        h->bins.tl = best_tl;
+        h->nbins = pmk::bin_count_of(n, best_tl);
+        h->bins.chunk = best_chunk;
+        h->max_chunks = pmk::deposit_max_chunks(n, best_tl, h->cap, best_chunk);
+        h->resort_interval = best_resort;
+        // the bin tables must match the restored particle order and the chosen tiles
+        if (st == PM_OK) st = sort_now(h, false);
+        // potential and force as the last field solve left them
+        if (st == PM_OK && fpar0 >= 0) {
+            h->rho_cur = fpar0;
+            st = stage_field(h);
+            h->rho_cur = rho_cur0;
+        }
+    }
+    h->profiling = prof;
+    if (st == PM_OK) PM_HIP(h, hipStreamSynchronize(h->stream));
+    return st;
+}
+
+extern "C" {
+
 
 pm_status pm_get_info(pm_handle h, pm_info* info) {
     if (!h || !info) return fail(h, PM_EINVAL, "null argument");

@@ -80,15 +80,34 @@ hipError_t push_deposit(int n, int mode, int wrap_mode, const float2* force, Par
 
 // ----- FFT field solve (pm_fft.hip) -------------------------------------------
 // Half spectrum layout: H[k * N + j], k in [0, N/2] (x frequency), j in [0, N)
-// (y row before the column pass, y frequency after it).
-hipError_t row_r2c(int n, const int64_t* rho, float inv_scale, const float2* tw, float2* H,
+// (y row before the column pass, y frequency after it).  Every pass takes an
+// array of `count` descriptors in device memory (one per simulation, all of
+// lattice size n) and launches grid.y = count.
+struct FieldDesc {
+    const int64_t* rho;   // mass lattice (int64 fixed point)
+    float2* H;            // half spectrum
+    float2* Hfy;          // half spectrum of the y derivative
+    const float* T;       // transfer function incl. 1/N^2, [kx][ky]
+    const float* dk;      // stencil symbol G*(8 sin t - sin 2t)/6
+    float* phi;           // potential
+    float2* force;        // (Fx, Fy)
+    float inv_scale;      // 2^-shift
+    float gravity;        // force multiplier
+};
+// Tile batch and radix of the three passes: rp row pairs per row-pass block,
+// cb columns per column-pass block, radix 4 or 8.
+struct FftPlan {
+    int rp = 2, cb = 1, radix = 4;
+};
+bool plan_valid(int n, FftPlan p);
+FftPlan default_plan(int n);
+hipError_t row_r2c(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s);
+hipError_t col_conv(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
+                    hipStream_t s);
+hipError_t row_c2r(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
                    hipStream_t s);
-hipError_t col_conv(int n, float2* H, float2* Hfy, const float* T, const float* dk,
-                    const float2* tw, bool with_force, hipStream_t s);
-hipError_t row_c2r(int n, const float2* H, const float2* Hfy, const float* dk, const float2* tw,
-                   float* phi, float2* force, bool with_force, hipStream_t s);
-hipError_t edge_force(int n, const float* phi, float g, float2* force, hipStream_t s);
-hipError_t stencil_force(int n, const float* phi, float g, float2* force, hipStream_t s);
+hipError_t edge_force(int n, const FieldDesc* d, int count, hipStream_t s);
+hipError_t stencil_force(int n, const FieldDesc* d, int count, hipStream_t s);
 
 // ----- FP64 setup spectra (pm_spectra.hip) --------------------------------------
 // T[k*N + ky] = Ghat(k,ky) * c/(c^2+eps) / N^2 as float, k in [0,N/2].

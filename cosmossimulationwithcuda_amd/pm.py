@@ -21,6 +21,7 @@ FORCE_SPECTRAL, FORCE_STENCIL = 0, 1
 FFT_FP32, FFT_BF16X3 = 0, 1
 LATTICE_MASS, LATTICE_POTENTIAL, LATTICE_FORCE, LATTICE_TRANSFER = range(4)
 STAGE_DEPOSIT, STAGE_FIELD, STAGE_PUSH = range(3)
+TUNE_FFT, TUNE_PARTICLES = 1, 2
 KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push", "sort", "push_deposit"]
 
 
@@ -92,6 +93,8 @@ def lib():
         L.pm_diagnostics.argtypes = [_H, ctypes.POINTER(ctypes.c_double)]
         L.pm_get_info.argtypes = [_H, ctypes.POINTER(Info)]
         L.pm_set_param.argtypes = [_H, ctypes.c_char_p, ctypes.c_int64]
+        L.pm_get_param.argtypes = [_H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
+        L.pm_autotune.argtypes = [_H, ctypes.c_int32, ctypes.c_int32]
         L.pm_profile.argtypes = [_H, ctypes.c_int32]
         L.pm_profile_read.argtypes = [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64)]
@@ -180,6 +183,17 @@ class Simulation:
 
     def set_param(self, name, value):
         _check(lib().pm_set_param(self._h, name.encode(), int(value)), self._h)
+
+    def get_param(self, name):
+        v = ctypes.c_int64()
+        _check(lib().pm_get_param(self._h, name.encode(), ctypes.byref(v)), self._h)
+        return v.value
+
+    def autotune(self, fft=True, particles=True, reps=5):
+        flags = (TUNE_FFT if fft else 0) | (TUNE_PARTICLES if particles else 0)
+        _check(lib().pm_autotune(self._h, flags, reps), self._h)
+        return {k: self.get_param(k) for k in ("fft_rp", "fft_cb", "fft_radix", "tile_log", "deposit_chunk",
+                                                 "resort_interval")}
 
     def info(self):
         i = Info()

@@ -150,10 +150,22 @@ pm_status pm_get_info(pm_handle h, pm_info* info);
 
 /* Runtime tunables that change speed, never results: "resort_interval"
  * (steps between particle re-binnings), "deposit_chunk" (particles per
- * particle-kernel block, a multiple of 512) and "tile_log" (bin edge 2^5 or
- * 2^6 cells; not below the lattice's default).  No counterpart in the reference, whose
+ * particle-kernel block, a multiple of 512), "tile_log" (bin edge 2^5 or
+ * 2^6 cells; not below the lattice's default) and the FFT plan "fft_rp",
+ * "fft_cb", "fft_radix" (see pm_autotune).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
+pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);
+
+/* Autotuner (no counterpart in the reference).  PM_TUNE_FFT times every
+ * compiled FFT plan ("fft_rp" row pairs per row-pass block in {1,2,4},
+ * "fft_cb" columns per column-pass block in {1,2,4,8}, "fft_radix" 4 or 8) on
+ * the current mass lattice and keeps the fastest; PM_TUNE_PARTICLES times
+ * tile size, chunk size and re-sort interval over `reps`-step trials on a
+ * device snapshot of the state, which is restored afterwards.  Results of
+ * later steps are bit-identical to an untuned run. */
+enum { PM_TUNE_FFT = 1, PM_TUNE_PARTICLES = 2 };
+pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps);
 
 /* Per-kernel device timing with HIP events on the handle's stream. */
 pm_status pm_profile(pm_handle h, int32_t enable);

@@ -355,3 +355,51 @@ def test_config2_full_size():
           f"(momentum scale {pscale:.3e}), KE {d['kinetic']:.4e}, PE {d['potential']:.4e}")
     assert d["mass"] == d0["mass"] and d["in_bounds"] == npart and d["nonfinite"] == 0
     assert abs(d["px"] - d0["px"]) < 1e-3 * pscale and abs(d["py"] - d0["py"]) < 1e-3 * pscale
+
+
+@pytest.mark.parametrize("radix", [4, 8])
+@pytest.mark.parametrize("n", [256, 512, 2048])
+def test_fft_plans_vs_fp64(n, radix):
+    """Every compiled FFT plan (row pairs, column batch, radix) meets the 1e-5
+    field bar; different plans agree to FP32 rounding."""
+    npart = 200000
+    x, y, vx, vy, m = ics.exponential_disk(npart, n / 2, n / 2, n / 16, n / 3, seed=41, masses="lognormal")
+    T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
+    with make_sim(n, npart, pm.CIC) as s:
+        s.load(x, y, vx, vy, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        phi64 = orc.potential(n, T, s.info().mass_shift, rho)
+        fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+        for rp in (1, 2, 4):
+            for cb in (1, 2, 4, 8):
+                try:
+                    s.set_param("fft_radix", radix); s.set_param("fft_rp", rp); s.set_param("fft_cb", cb)
+                except pm.PMError:
+                    continue        # plan not compiled for this lattice (LDS limit)
+                s.stage(pm.STAGE_FIELD)
+                phi = s.read_lattice(pm.LATTICE_POTENTIAL)
+                force = s.read_lattice(pm.LATTICE_FORCE)
+                e_phi = rel_l2(phi, phi64)
+                e_f = float(np.sqrt(np.sum((force[..., 0] - fx64) ** 2 + (force[..., 1] - fy64) ** 2)
+                                    / np.sum(fx64 ** 2 + fy64 ** 2)))
+                print(f"N={n} radix={radix} rp={rp} cb={cb}: phi {e_phi:.2e} force {e_f:.2e}")
+                assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD
+
+
+def test_autotune_keeps_state():
+    """The autotuner picks a plan and particle parameters without changing the
+    simulation: states after 5 more steps are bit-identical to an untuned run."""
+    n, npart = 512, 400000
+    x, y, vx, vy, m = ics.exponential_disk(npart, 256, 256, 30, 200, seed=42, masses="lognormal")
+    outs = []
+    for tune in (False, True):
+        with make_sim(n, npart, pm.CIC, dt=0.05) as s:
+            s.load(x, y, vx, vy, m)
+            s.step(3)
+            if tune:
+                print("autotune:", s.autotune(reps=2))
+            s.step(5)
+            outs.append(s.read())
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -148,11 +148,12 @@ pm_status pm_diagnostics(pm_handle h, double out[8]);
 
 pm_status pm_get_info(pm_handle h, pm_info* info);
 
-/* Runtime tunables that change speed, never results: "resort_interval"
+/* Runtime tunables: "resort_interval"
  * (steps between particle re-binnings), "deposit_chunk" (particles per
  * particle-kernel block, a multiple of 512), "tile_log" (bin edge 2^5 or
- * 2^6 cells; not below the lattice's default) and the FFT plan "fft_rp",
- * "fft_cb", "fft_radix" (see pm_autotune).  No counterpart in the reference, whose
+ * 2^6 cells; not below the lattice's default) change speed, never results;
+ * the FFT plan "fft_rp", "fft_cb", "fft_radix" changes FP32 rounding only
+ * (see pm_autotune).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
 pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);
@@ -162,8 +163,9 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);
  * "fft_cb" columns per column-pass block in {1,2,4,8}, "fft_radix" 4 or 8) on
  * the current mass lattice and keeps the fastest; PM_TUNE_PARTICLES times
  * tile size, chunk size and re-sort interval over `reps`-step trials on a
- * device snapshot of the state, which is restored afterwards.  Results of
- * later steps are bit-identical to an untuned run. */
+ * device snapshot of the state, which is restored afterwards.  The particle
+ * parameters never change results; a different FFT plan changes the FP32
+ * rounding of the field (every plan meets the 1e-5 relative L2 bar). */
 enum { PM_TUNE_FFT = 1, PM_TUNE_PARTICLES = 2 };
 pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps);
 

@@ -388,18 +388,22 @@ def test_fft_plans_vs_fp64(n, radix):
 
 
 def test_autotune_keeps_state():
-    """The autotuner picks a plan and particle parameters without changing the
-    simulation: states after 5 more steps are bit-identical to an untuned run."""
+    """The autotuner leaves the simulation state untouched: after tuning the
+    particle parameters, 5 more steps are bit-identical to an untuned run;
+    after also retuning the FFT plan they agree to FP32 field rounding."""
     n, npart = 512, 400000
     x, y, vx, vy, m = ics.exponential_disk(npart, 256, 256, 30, 200, seed=42, masses="lognormal")
     outs = []
-    for tune in (False, True):
+    for tune in (None, "particles", "all"):
         with make_sim(n, npart, pm.CIC, dt=0.05) as s:
             s.load(x, y, vx, vy, m)
             s.step(3)
             if tune:
-                print("autotune:", s.autotune(reps=2))
+                print(tune, "autotune:", s.autotune(fft=(tune == "all"), particles=True, reps=2))
             s.step(5)
             outs.append(s.read())
-    for a, b in zip(*outs):
+    for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    d = np.hypot(outs[2][0] - outs[0][0], outs[2][1] - outs[0][1])
+    print(f"FFT plan retuned: max |dx| after 5 steps {d.max():.2e} cells")
+    assert d.max() < 1e-3

@@ -15,58 +15,15 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pm.h"
-#include "pm_internal.h"
+#include "pm_sim.h"
 
 namespace {
 
 thread_local std::string g_last_error;   // errors of calls made without a valid handle
 
-struct ProfRec {
-    int kernel;
-    hipEvent_t start, end;
-};
-
 }  // namespace
 
-struct pm_sim {
-    pm_config cfg{};
-    hipStream_t stream = nullptr;
-    int64_t cap = 0, np = 0;
-    int shift = 40;
-    pmk::ParticleArrays cur{}, alt{};      // sorted state and the counting-sort target / read scratch
-    pmk::BinState bins{};
-    int nbins = 0;                         // in-bounds bins + parked bin
-    int nbins_alloc = 0;                   // bins the bin arrays were sized for
-    int max_chunks = 0;
-    int resort_interval = 8;
-    int steps_since_sort = 0;
-    int64_t sorts = 0;
-    unsigned long long* migrants = nullptr;
-    int64_t* rho_buf[2] = {nullptr, nullptr};
-    int rho_cur = 0;                       // rho_buf[rho_cur]: mass lattice of the current positions (when rho_valid)
-    int field_parity = -1;                 // rho_buf index the last field solve consumed (potential energy)
-    pmk::FieldDesc* fdesc = nullptr;       // device descriptors, [p] reads rho_buf[p]
-    pmk::FftPlan plan{};
-    bool rho_valid = false;
-    float2 *H = nullptr, *Hfy = nullptr;
-    float* T = nullptr;
-    float* dk = nullptr;
-    float2* tw = nullptr;
-    float* phi = nullptr;
-    float2* force = nullptr;
-    double* diag = nullptr;
-    int64_t bytes = 0;
-    std::vector<void*> allocs;
-    std::string err;
-    bool profiling = false;
-    std::vector<ProfRec> recs;
-    std::vector<hipEvent_t> pool;
-    double prof_ms[PM_K_COUNT] = {};
-    int64_t prof_n[PM_K_COUNT] = {};
-};
-
-namespace {
+namespace pmi {
 
 pm_status fail(pm_sim* h, pm_status st, const std::string& msg) {
     if (h) h->err = msg;
@@ -79,11 +36,6 @@ pm_status hip_fail(pm_sim* h, hipError_t e, const char* what) {
     return fail(h, e == hipErrorOutOfMemory ? PM_ENOMEM : PM_EDEVICE, msg);
 }
 
-#define PM_HIP(h, call)                                        \
-    do {                                                       \
-        hipError_t e_ = (call);                                \
-        if (e_ != hipSuccess) return hip_fail((h), e_, #call); \
-    } while (0)
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
@@ -106,7 +58,7 @@ pm_status validate(const pm_config* c, std::string* why) {
 int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
 struct Layout {
-    int64_t particles, bins, rho, H, T, dk, tw, phi, force, diag, total, setup_work;
+    int64_t particles, bins, rho, fdesc, H, T, dk, tw, phi, force, diag, total, setup_work;
 };
 
 Layout layout_of(const pm_config* c) {
@@ -124,7 +76,9 @@ Layout layout_of(const pm_config* c) {
     L.force = align256(8 * n * n);
     const int64_t dblocks = std::max(pmk::diag_blocks(cap), pmk::diag_lattice_blocks((int)n));
     L.diag = align256(8 * 8 * dblocks);
-    L.total = 12 * L.particles + L.bins + 2 * L.rho + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force + L.diag;
+    L.fdesc = align256(2 * sizeof(pmk::FieldDesc));
+    L.total = 12 * L.particles + L.bins + 2 * L.rho + L.fdesc + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force +
+              L.diag;
     L.setup_work = 16 * n * n;
     return L;
 }
@@ -321,7 +275,77 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
     return upload_field_desc(h);   // inv_scale follows the shift
 }
 
-}  // namespace
+// Stencil symbol d(k) = G (8 sin t - sin 2t)/6, t = 2 pi k/N (exactly 0 at
+// k = 0 and k = N/2), and twiddles exp(-2 pi i k/N), both evaluated in FP64.
+pm_status upload_tables(pm_sim* h) {
+    const int n = h->cfg.n_lattice;
+    std::vector<float2> tw(n);
+    std::vector<float> dk(n);
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < n; ++k) {
+        const double a = 2.0 * pi * k / n;
+        tw[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+        dk[k] = (k == 0 || k == n / 2)
+                    ? 0.0f
+                    : (float)((double)h->cfg.gravity * (8.0 * std::sin(a) - std::sin(2.0 * a)) / 6.0);
+    }
+    PM_HIP(h, hipMemcpyAsync(h->tw, tw.data(), 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->dk, dk.data(), 4 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
+pm_status build_spectra(pm_sim* h) {
+    const int n = h->cfg.n_lattice;
+    double* work = nullptr;
+    hipError_t e = hipMalloc(&work, (size_t)16 * n * n);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMalloc(setup work)");
+    e = pmk::build_transfer(n, h->cfg.mode, h->cfg.green_origin, h->cfg.deconv_eps, work, h->T, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(work);
+    if (e != hipSuccess) return hip_fail(h, e, "build_transfer");
+    if (e2 != hipSuccess) return hip_fail(h, e2, "build_transfer sync");
+    return PM_OK;
+}
+
+pm_status reconfigure(pm_sim* h, const pm_config* cfg) {
+    std::string why;
+    if (validate(cfg, &why) != PM_OK) return fail(h, PM_EINVAL, why);
+    if (cfg->n_lattice != h->cfg.n_lattice || cfg->n_particles > h->cap || cfg->device != h->cfg.device)
+        return fail(h, PM_EINVAL, "reconfigure: lattice, device or capacity differ");
+    const bool spectra = cfg->mode != h->cfg.mode || cfg->green_origin != h->cfg.green_origin ||
+                         cfg->deconv_eps != h->cfg.deconv_eps;
+    const bool tables = cfg->gravity != h->cfg.gravity;
+    const int64_t cap = h->cap;
+    h->cfg = *cfg;
+    h->cfg.n_particles = cap;
+    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
+    pm_status st = PM_OK;
+    if (spectra && (st = build_spectra(h)) != PM_OK) return st;
+    if (tables && (st = upload_tables(h)) != PM_OK) return st;
+    h->np = 0;
+    h->rho_valid = false;
+    h->field_parity = -1;
+    return upload_field_desc(h);
+}
+
+HostState save_state(const pm_sim* h) {
+    return HostState{h->cur, h->alt, h->rho_cur, h->field_parity, h->steps_since_sort, h->rho_valid, h->sorts};
+}
+
+void restore_state(pm_sim* h, const HostState& s) {
+    h->cur = s.cur;
+    h->alt = s.alt;
+    h->rho_cur = s.rho_cur;
+    h->field_parity = s.field_parity;
+    h->steps_since_sort = s.steps_since_sort;
+    h->rho_valid = s.rho_valid;
+    h->sorts = s.sorts;
+}
+
+}  // namespace pmi
+
+using namespace pmi;
 
 extern "C" {
 
@@ -403,7 +427,7 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     }
     if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_buf[0] = static_cast<int64_t*>(p);
     if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_buf[1] = static_cast<int64_t*>(p);
-    if ((st = dalloc(h, &p, align256(2 * sizeof(pmk::FieldDesc)))) != PM_OK) return bail(st);
+    if ((st = dalloc(h, &p, L.fdesc)) != PM_OK) return bail(st);
     h->fdesc = static_cast<pmk::FieldDesc*>(p);
     h->plan = pmk::default_plan(n);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
@@ -415,30 +439,8 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     if ((st = dalloc(h, &p, L.force)) != PM_OK) return bail(st); h->force = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.diag)) != PM_OK) return bail(st); h->diag = static_cast<double*>(p);
 
-    // Twiddles exp(-2 pi i m/N) and the stencil symbol d(k) = G (8 sin t - sin 2t)/6,
-    // t = 2 pi k/N (exactly 0 at k = 0 and k = N/2), both evaluated in FP64.
-    std::vector<float2> tw(n);
-    std::vector<float> dk(n);
-    const double pi = 3.14159265358979323846;
-    for (int k = 0; k < n; ++k) {
-        const double a = 2.0 * pi * k / n;
-        tw[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
-        dk[k] = (k == 0 || k == n / 2) ? 0.0f
-                                       : (float)((double)cfg->gravity * (8.0 * std::sin(a) - std::sin(2.0 * a)) / 6.0);
-    }
-    if (hipMemcpyAsync(h->tw, tw.data(), 8 * (size_t)n, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(h->dk, dk.data(), 4 * (size_t)n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
-        return bail(fail(h, PM_EDEVICE, "twiddle upload failed"));
-
-    // FP64 transfer function (setup scratch freed right after)
-    double* work = nullptr;
-    hipError_t e = hipMalloc(&work, (size_t)L.setup_work);
-    if (e != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(setup work)"));
-    e = pmk::build_transfer(n, cfg->mode, cfg->green_origin, cfg->deconv_eps, work, h->T, h->stream);
-    hipError_t e2 = hipStreamSynchronize(h->stream);
-    (void)hipFree(work);
-    if (e != hipSuccess) return bail(hip_fail(h, e, "build_transfer"));
-    if (e2 != hipSuccess) return bail(hip_fail(h, e2, "build_transfer sync"));
+    if ((st = upload_tables(h)) != PM_OK) return bail(st);
+    if ((st = build_spectra(h)) != PM_OK) return bail(st);
     // deterministic initial lattices (force/potential read before any step are zero)
     if (pmk::fill_zero(h->phi, (size_t)4 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||

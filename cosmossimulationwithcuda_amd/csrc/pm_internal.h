@@ -109,6 +109,32 @@ hipError_t row_c2r(int n, const FieldDesc* d, int count, const float2* tw, FftPl
 hipError_t edge_force(int n, const FieldDesc* d, int count, hipStream_t s);
 hipError_t stencil_force(int n, const FieldDesc* d, int count, hipStream_t s);
 
+// ----- batched small simulations (pm_particles.hip, pm_util.hip) ------------------
+// One fused push + deposit launch over many simulations whose lattices are
+// small enough to stay in L2: no tile windows, force gathered and mass
+// deposited with global memory operations.  Particle order is irrelevant.
+struct PartDesc {
+    float *x, *y, *vx, *vy;
+    const float* m;
+    const float2* force;
+    unsigned long long* rho_next;   // zero on entry
+    int64_t np;
+    int n;
+    float dt;
+    float scale;                    // 2^shift
+    int wrap_mode;
+};
+// block_start[i]: first block of simulation i (exclusive prefix of
+// ceil(np_i / (4 * kBlock)) blocks), block_start[count] = total blocks.
+hipError_t push_deposit_direct(int mode, const PartDesc* d, const int* block_start, int count, int total_blocks,
+                               hipStream_t s);
+int direct_blocks(int64_t np);
+struct FillDesc {
+    void* p;
+    size_t bytes;   // multiple of 16
+};
+hipError_t fill_zero_batched(const FillDesc* d, int count, size_t max_bytes, hipStream_t s);
+
 // ----- FP64 setup spectra (pm_spectra.hip) --------------------------------------
 // T[k*N + ky] = Ghat(k,ky) * c/(c^2+eps) / N^2 as float, k in [0,N/2].
 // work64 must hold 2 * N * N doubles (complex lattice).

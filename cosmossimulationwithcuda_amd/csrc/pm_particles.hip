@@ -373,6 +373,55 @@ __global__ __launch_bounds__(kBlock) void k_push_tail(const float2* __restrict__
     vy[p] = uy;
 }
 
+// ------------------------------------------------- batched direct push + deposit --
+// Four consecutive particles per lane of one simulation of the batch (block
+// -> simulation by binary search over block_start); force from the lattice,
+// deposit with 64-bit global atomics.  Same arithmetic as the tiled kernels.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_push_deposit_direct(const PartDesc* __restrict__ descs,
+                                                                 const int* __restrict__ block_start, int count) {
+    int lo = 0, hi = count;                  // last i with block_start[i] <= blockIdx.x
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (block_start[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+    }
+    const PartDesc d = descs[lo];
+    const int64_t q = (int64_t)(blockIdx.x - block_start[lo]) * kBlock + threadIdx.x;   // group of 4 slots
+    if (4 * q >= d.np) return;
+    const float fhi = (float)(d.n - 1);
+    // capacity is padded to 64 slots, so the last group reads in bounds
+    v4f X = reinterpret_cast<const v4f*>(d.x)[q], Y = reinterpret_cast<const v4f*>(d.y)[q];
+    v4f VX = reinterpret_cast<const v4f*>(d.vx)[q], VY = reinterpret_cast<const v4f*>(d.vy)[q];
+    const v4f M = reinterpret_cast<const v4f*>(d.m)[q];
+    float px[4] = {X.x, X.y, X.z, X.w}, py[4] = {Y.x, Y.y, Y.z, Y.w};
+    float ux[4] = {VX.x, VX.y, VX.z, VX.w}, uy[4] = {VY.x, VY.y, VY.z, VY.w};
+    const float pm[4] = {M.x, M.y, M.z, M.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (4 * q + k >= d.np) break;
+        if (d.wrap_mode) push_one<MODE, 1>(d.force, d.n, d.dt, px[k], py[k], ux[k], uy[k]);
+        else push_one<MODE, 0>(d.force, d.n, d.dt, px[k], py[k], ux[k], uy[k]);
+    }
+    if (4 * q + 3 < d.np) {
+        reinterpret_cast<v4f*>(d.x)[q] = v4f{px[0], px[1], px[2], px[3]};
+        reinterpret_cast<v4f*>(d.y)[q] = v4f{py[0], py[1], py[2], py[3]};
+        reinterpret_cast<v4f*>(d.vx)[q] = v4f{ux[0], ux[1], ux[2], ux[3]};
+        reinterpret_cast<v4f*>(d.vy)[q] = v4f{uy[0], uy[1], uy[2], uy[3]};
+    } else {
+        for (int k = 0; k < 4 && 4 * q + k < d.np; ++k) {
+            d.x[4 * q + k] = px[k];
+            d.y[4 * q + k] = py[k];
+            d.vx[4 * q + k] = ux[k];
+            d.vy[4 * q + k] = uy[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (4 * q + k >= d.np) break;
+        if (in_bounds(px[k], py[k], fhi)) deposit_global<MODE>(d.rho_next, d.n, px[k], py[k], pm[k], d.scale);
+    }
+}
+
 // ------------------------------------------------------------------ launchers --
 template <int TL>
 hipError_t deposit_t(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
@@ -461,6 +510,18 @@ hipError_t push_deposit(int n, int mode, int wrap_mode, const float2* force, Par
         return push_deposit_mode<5>(n, mode, wrap_mode, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins,
                                     s);
     return push_deposit_mode<6>(n, mode, wrap_mode, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, s);
+}
+
+int direct_blocks(int64_t np) { return (int)((np + 4 * kBlock - 1) / (4 * kBlock)); }
+
+hipError_t push_deposit_direct(int mode, const PartDesc* d, const int* block_start, int count, int total_blocks,
+                               hipStream_t s) {
+    if (total_blocks <= 0 || count <= 0) return hipSuccess;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_push_deposit_direct<0>), dim3(total_blocks), dim3(kBlock), 0, s, d, block_start, count);
+    else
+        hipLaunchKernelGGL((k_push_deposit_direct<1>), dim3(total_blocks), dim3(kBlock), 0, s, d, block_start, count);
+    return hipGetLastError();
 }
 
 hipError_t push(int n, int mode, const float2* force, float* x, float* y, float* vx, float* vy, int64_t np,

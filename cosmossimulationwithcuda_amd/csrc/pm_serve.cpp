@@ -1,0 +1,745 @@
+// Serving engine (include/pm_serve.h): continuous admission of independent
+// simulation requests under a device-memory budget, device-resident state,
+// rounds that advance every admitted request together, batched launches for
+// small lattices, captured HIP graphs for large ones, latency statistics.
+// No counterpart in the reference (its host side runs one simulation per GPU,
+// CosmosCuda.cuh:1302-1395).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/pm_serve.h"
+#include "pm_sim.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
+
+int64_t capacity_class(int64_t n) {
+    int64_t c = 1024;
+    while (c < n) c <<= 1;
+    return c;
+}
+
+struct Request {
+    int64_t id = 0;
+    pm_config cfg{};
+    int64_t n = 0;
+    int32_t steps = 0;
+    std::vector<float> x, y, vx, vy, m;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int32_t done_steps = 0;
+    int32_t state = PM_REQ_QUEUED;
+    std::string msg;
+    pm_sim* sim = nullptr;
+    bool batched = false;
+    // captured rounds of a large simulation: graph[k] advances from state[k] to state[k ^ 1]
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    pmi::HostState gstate[2];
+    int64_t sorts_per_round = 0;
+    int gphase = 0;
+    bool graphs_tried = false;
+    int rounds_done = 0;
+};
+
+void destroy_graphs(Request* r) {
+    for (auto& g : r->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    r->graphs_tried = false;
+}
+
+template <typename T>
+double percentile(std::vector<T> v, double q) {
+    if (v.empty()) return 0.0;
+    const size_t k = (size_t)std::min<double>((double)v.size() - 1, std::floor(q * (double)(v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + k, v.end());
+    return (double)v[k];
+}
+
+// Descriptor tables of one batched round, laid out in one device buffer.
+struct BatchPlan {
+    std::vector<int64_t> signature;                  // request ids of the members
+    std::vector<Request*> members;
+    struct NGroup {
+        int n;
+        std::vector<int> idx;
+        size_t fdesc_off[2];
+    };
+    struct MGroup {
+        int mode;
+        std::vector<int> idx;
+        size_t pdesc_off[2];
+        size_t bstart_off;
+        int total_blocks;
+    };
+    std::vector<NGroup> ngroups;
+    std::vector<MGroup> mgroups;
+    size_t fill_off[2] = {0, 0};
+    size_t max_fill = 0;
+    hipGraphExec_t graph = nullptr;
+};
+
+}  // namespace
+
+struct pm_server {
+    pm_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Request*> active;
+    std::vector<pm_sim*> pool;
+    int64_t bytes_in_use = 0, peak_bytes = 0;
+    std::thread worker;
+    bool stop = false;
+    int64_t next_id = 1;
+    std::string err, err_copy;   // last failure reported to callers (guarded by mu)
+    std::string batch_err;       // worker only
+    // statistics (guarded by mu)
+    std::vector<float> step_lat_ms, req_lat_ms;
+    double particle_steps = 0, steps_done = 0, t_first_submit = -1, t_last_done = 0;
+    int64_t submitted = 0, completed = 0, failed = 0;
+    int64_t rounds = 0, batched_rounds = 0, graph_launches = 0, graph_captures = 0, pool_hits = 0;
+    // batched launches
+    hipStream_t batch_stream = nullptr;
+    void* dscratch = nullptr;
+    size_t dscratch_bytes = 0;
+    void* hscratch = nullptr;
+    BatchPlan plan;
+};
+
+namespace {
+
+int64_t sim_bytes(const pm_config& c) {
+    pm_config k = c;
+    k.n_particles = capacity_class(c.n_particles);
+    return pm_instance_bytes(&k);
+}
+
+void fail_request(pm_server* s, Request* r, std::string msg) {   // by value: msg may live in r->sim
+    if (r->sim) {
+        destroy_graphs(r);
+        s->bytes_in_use -= r->sim->bytes;
+        pm_destroy(r->sim);
+        r->sim = nullptr;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->state = PM_REQ_FAILED;
+    r->msg = std::move(msg);
+    r->t_done = now_s();
+    s->failed += 1;
+    s->cv_done.notify_all();
+}
+
+// A pooled simulation of the same lattice size, device, and a capacity within
+// 2x of what the request needs; nullptr if none.
+pm_sim* take_pooled(pm_server* s, const Request* r) {
+    const int64_t need = capacity_class(r->n);
+    for (size_t i = 0; i < s->pool.size(); ++i) {
+        pm_sim* h = s->pool[i];
+        if (h->cfg.n_lattice == r->cfg.n_lattice && h->cap >= r->n && h->cap <= 2 * need) {
+            s->pool.erase(s->pool.begin() + (long)i);
+            return h;
+        }
+    }
+    return nullptr;
+}
+
+// Admit from the queue head while memory and slots allow (strict FIFO).
+void admit(pm_server* s) {
+    const int max_active = s->cfg.mode == PM_SERVE_PLAIN ? 1 : std::max(1, s->cfg.max_active);
+    while ((int)s->active.size() < max_active) {
+        Request* r;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->queue.empty()) return;
+            r = s->queue.front();
+        }
+        pm_sim* h = take_pooled(s, r);
+        if (h) {
+            s->pool_hits += 1;
+        } else {
+            const int64_t need = sim_bytes(r->cfg);
+            // make room by releasing idle pooled simulations, largest first
+            std::sort(s->pool.begin(), s->pool.end(), [](pm_sim* a, pm_sim* b) { return a->bytes < b->bytes; });
+            while (s->bytes_in_use + need > s->cfg.hbm_budget_bytes && !s->pool.empty()) {
+                pm_sim* v = s->pool.back();
+                s->pool.pop_back();
+                s->bytes_in_use -= v->bytes;
+                pm_destroy(v);
+            }
+            if (pm_plan_admission(&need, 1, s->cfg.hbm_budget_bytes - s->bytes_in_use, 1) == 0) return;
+            pm_config c = r->cfg;
+            c.n_particles = capacity_class(r->n);
+            c.device = s->cfg.device;
+            if (pm_create(&c, &h) != PM_OK) {
+                {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->queue.pop_front();
+                }
+                fail_request(s, r, std::string("create: ") + pm_last_error(nullptr));
+                continue;
+            }
+            s->bytes_in_use += h->bytes;
+            s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_use);
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->queue.pop_front();
+        }
+        r->sim = h;
+        pm_config c = r->cfg;
+        c.n_particles = h->cap;
+        c.device = s->cfg.device;
+        pm_status st = pmi::reconfigure(h, &c);
+        if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);
+        if (st == PM_OK) st = pmi::stage_deposit(h);
+        if (st == PM_OK) st = pm_sync(h);
+        if (st != PM_OK) {
+            fail_request(s, r, h->err);
+            continue;
+        }
+        r->batched = s->cfg.mode == PM_SERVE_BATCHED && r->cfg.n_lattice <= s->cfg.batch_max_lattice &&
+                     r->cfg.force_mode == PM_FORCE_SPECTRAL;
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->t_admit = now_s();
+        r->state = PM_REQ_ACTIVE;
+        s->active.push_back(r);
+    }
+}
+
+// ---- batched launches -------------------------------------------------------
+
+pmk::FieldDesc field_desc(const pm_sim* h, int rho_index) {
+    pmk::FieldDesc d;
+    d.rho = h->rho_buf[rho_index];
+    d.H = h->H;
+    d.Hfy = h->Hfy;
+    d.T = h->T;
+    d.dk = h->dk;
+    d.phi = h->phi;
+    d.force = h->force;
+    d.inv_scale = std::ldexp(1.0f, -h->shift);
+    d.gravity = h->cfg.gravity;
+    return d;
+}
+
+bool ensure_scratch(pm_server* s, size_t bytes) {
+    if (bytes <= s->dscratch_bytes) return true;
+    if (s->dscratch) (void)hipFree(s->dscratch);
+    if (s->hscratch) (void)hipHostFree(s->hscratch);
+    s->dscratch = nullptr;
+    s->hscratch = nullptr;
+    s->dscratch_bytes = 0;
+    size_t b = 1 << 16;
+    while (b < bytes) b <<= 1;
+    if (hipMalloc(&s->dscratch, b) != hipSuccess) return false;
+    if (hipHostMalloc(&s->hscratch, b, hipHostMallocDefault) != hipSuccess) return false;
+    s->dscratch_bytes = b;
+    return true;
+}
+
+// Builds and uploads the descriptor tables for the batched members; tables
+// for step parity p address rho_buf[rho_cur ^ p] as the current lattice.
+pm_status build_plan(pm_server* s, const std::vector<Request*>& members) {
+    BatchPlan& P = s->plan;
+    if (P.graph) { (void)hipGraphExecDestroy(P.graph); P.graph = nullptr; }
+    P.members = members;
+    P.signature.clear();
+    for (Request* r : members) P.signature.push_back(r->id);
+    P.ngroups.clear();
+    P.mgroups.clear();
+    for (int i = 0; i < (int)members.size(); ++i) {
+        const pm_sim* h = members[i]->sim;
+        auto ng = std::find_if(P.ngroups.begin(), P.ngroups.end(),
+                               [&](const BatchPlan::NGroup& g) { return g.n == h->cfg.n_lattice; });
+        if (ng == P.ngroups.end()) P.ngroups.push_back({h->cfg.n_lattice, {i}, {0, 0}});
+        else ng->idx.push_back(i);
+        auto mg = std::find_if(P.mgroups.begin(), P.mgroups.end(),
+                               [&](const BatchPlan::MGroup& g) { return g.mode == h->cfg.mode; });
+        if (mg == P.mgroups.end()) P.mgroups.push_back({h->cfg.mode, {i}, {0, 0}, 0, 0});
+        else mg->idx.push_back(i);
+    }
+    // layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off = (off + bytes + 255) & ~(size_t)255; return o; };
+    for (auto& g : P.ngroups)
+        for (int p = 0; p < 2; ++p) g.fdesc_off[p] = take(sizeof(pmk::FieldDesc) * g.idx.size());
+    for (auto& g : P.mgroups) {
+        for (int p = 0; p < 2; ++p) g.pdesc_off[p] = take(sizeof(pmk::PartDesc) * g.idx.size());
+        g.bstart_off = take(sizeof(int) * (g.idx.size() + 1));
+    }
+    for (int p = 0; p < 2; ++p) P.fill_off[p] = take(sizeof(pmk::FillDesc) * members.size());
+    if (!ensure_scratch(s, off)) {
+        s->batch_err = "scratch allocation failed";
+        return PM_ENOMEM;
+    }
+    char* hb = static_cast<char*>(s->hscratch);
+    for (auto& g : P.ngroups)
+        for (int p = 0; p < 2; ++p) {
+            auto* d = reinterpret_cast<pmk::FieldDesc*>(hb + g.fdesc_off[p]);
+            for (size_t k = 0; k < g.idx.size(); ++k) {
+                const pm_sim* h = members[g.idx[k]]->sim;
+                d[k] = field_desc(h, h->rho_cur ^ p);
+            }
+        }
+    for (auto& g : P.mgroups) {
+        int* bs = reinterpret_cast<int*>(hb + g.bstart_off);
+        int total = 0;
+        for (int p = 0; p < 2; ++p) {
+            auto* d = reinterpret_cast<pmk::PartDesc*>(hb + g.pdesc_off[p]);
+            for (size_t k = 0; k < g.idx.size(); ++k) {
+                const pm_sim* h = members[g.idx[k]]->sim;
+                pmk::PartDesc& q = d[k];
+                q.x = h->cur.x;
+                q.y = h->cur.y;
+                q.vx = h->cur.vx;
+                q.vy = h->cur.vy;
+                q.m = h->cur.m;
+                q.force = h->force;
+                q.rho_next = reinterpret_cast<unsigned long long*>(h->rho_buf[h->rho_cur ^ p ^ 1]);
+                q.np = h->np;
+                q.n = h->cfg.n_lattice;
+                q.dt = h->cfg.dt;
+                q.scale = std::ldexp(1.0f, h->shift);
+                q.wrap_mode = h->cfg.wrap_mode;
+            }
+        }
+        for (size_t k = 0; k < g.idx.size(); ++k) {
+            bs[k] = total;
+            total += pmk::direct_blocks(members[g.idx[k]]->sim->np);
+        }
+        bs[g.idx.size()] = total;
+        g.total_blocks = total;
+    }
+    P.max_fill = 0;
+    for (int p = 0; p < 2; ++p) {
+        auto* d = reinterpret_cast<pmk::FillDesc*>(hb + P.fill_off[p]);
+        for (size_t k = 0; k < members.size(); ++k) {
+            const pm_sim* h = members[k]->sim;
+            const size_t b = (size_t)8 * h->cfg.n_lattice * h->cfg.n_lattice;
+            d[k] = {h->rho_buf[h->rho_cur ^ p ^ 1], b};
+            P.max_fill = std::max(P.max_fill, b);
+        }
+    }
+    if (hipMemcpyAsync(s->dscratch, s->hscratch, off, hipMemcpyHostToDevice, s->batch_stream) != hipSuccess ||
+        hipStreamSynchronize(s->batch_stream) != hipSuccess) {
+        s->batch_err = "descriptor upload failed";
+        return PM_EDEVICE;
+    }
+    return PM_OK;
+}
+
+pm_status launch_batched_steps(pm_server* s, int steps) {
+    BatchPlan& P = s->plan;
+    char* db = static_cast<char*>(s->dscratch);
+    for (int st = 0; st < steps; ++st) {
+        const int p = st & 1;
+        if (pmk::fill_zero_batched(reinterpret_cast<pmk::FillDesc*>(db + P.fill_off[p]), (int)P.members.size(),
+                                   P.max_fill, s->batch_stream) != hipSuccess)
+            return PM_EDEVICE;
+        for (auto& g : P.ngroups) {
+            const pm_sim* h0 = P.members[g.idx[0]]->sim;
+            const auto* d = reinterpret_cast<const pmk::FieldDesc*>(db + g.fdesc_off[p]);
+            const int cnt = (int)g.idx.size();
+            if (pmk::row_r2c(g.n, d, cnt, h0->tw, h0->plan, s->batch_stream) != hipSuccess ||
+                pmk::col_conv(g.n, d, cnt, h0->tw, h0->plan, true, s->batch_stream) != hipSuccess ||
+                pmk::row_c2r(g.n, d, cnt, h0->tw, h0->plan, true, s->batch_stream) != hipSuccess ||
+                pmk::edge_force(g.n, d, cnt, s->batch_stream) != hipSuccess)
+                return PM_EDEVICE;
+        }
+        for (auto& g : P.mgroups) {
+            if (pmk::push_deposit_direct(g.mode, reinterpret_cast<const pmk::PartDesc*>(db + g.pdesc_off[p]),
+                                         reinterpret_cast<const int*>(db + g.bstart_off), (int)g.idx.size(),
+                                         g.total_blocks, s->batch_stream) != hipSuccess)
+                return PM_EDEVICE;
+        }
+    }
+    return PM_OK;
+}
+
+// Host-side state after `steps` batched steps (even): the lattice parity is
+// back where it started, the lattice of the new positions is valid.
+// The batched kernels ignore particle order, so the tile sort is not run; a
+// later tiled step (the request's tail) counts bins and re-sorts right away.
+void advance_batched_state(pm_sim* h, int steps) {
+    h->rho_cur ^= (steps & 1);
+    h->rho_valid = true;
+    h->field_parity = h->rho_cur ^ 1;
+    h->steps_since_sort = std::max(0, h->resort_interval - 1);
+}
+
+pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, int steps) {
+    std::vector<int64_t> sig;
+    for (Request* r : members) sig.push_back(r->id);
+    pm_status st = PM_OK;
+    if (sig != s->plan.signature || s->plan.members.empty()) {
+        if ((st = build_plan(s, members)) != PM_OK) return st;
+    }
+    if (s->cfg.use_graphs) {
+        if (!s->plan.graph) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(s->batch_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return PM_EDEVICE;
+            st = launch_batched_steps(s, steps);
+            const hipError_t e = hipStreamEndCapture(s->batch_stream, &g);
+            if (st != PM_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return PM_EDEVICE;
+            }
+            const hipError_t e2 = hipGraphInstantiate(&s->plan.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e2 != hipSuccess) { s->plan.graph = nullptr; return PM_EDEVICE; }
+            s->graph_captures += 1;
+        }
+        if (hipGraphLaunch(s->plan.graph, s->batch_stream) != hipSuccess) return PM_EDEVICE;
+        s->graph_launches += 1;
+    } else {
+        if ((st = launch_batched_steps(s, steps)) != PM_OK) return st;
+    }
+    for (Request* r : members) advance_batched_state(r->sim, steps);
+    s->batched_rounds += 1;
+    return PM_OK;
+}
+
+// A large simulation's round of `steps` steps: replayed from one of two
+// captured graphs (its re-sort swaps the particle buffers, so consecutive
+// rounds alternate) when graphs are on and the round lines up with the
+// re-sort cadence; eager otherwise.  The first round runs eagerly so that
+// launch attributes set lazily by the launchers are in place before capture.
+pm_status run_large_round(pm_server* s, Request* r, int steps) {
+    pm_sim* h = r->sim;
+    r->rounds_done += 1;
+    if (s->cfg.use_graphs && !r->graphs_tried && r->rounds_done > 1) {
+        r->graphs_tried = true;
+        if (h->steps_since_sort == 0 && h->resort_interval == steps && !h->profiling) {
+            const pmi::HostState s0 = pmi::save_state(h);
+            bool ok = true;
+            for (int k = 0; k < 2 && ok; ++k) {
+                r->gstate[k] = pmi::save_state(h);
+                hipGraph_t g = nullptr;
+                ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+                if (!ok) break;
+                const pm_status st = pm_step(h, steps);
+                const hipError_t e = hipStreamEndCapture(h->stream, &g);
+                ok = st == PM_OK && e == hipSuccess &&
+                     hipGraphInstantiate(&r->graph[k], g, nullptr, nullptr, 0) == hipSuccess;
+                if (g) (void)hipGraphDestroy(g);
+            }
+            pmi::restore_state(h, s0);
+            if (ok) {
+                r->sorts_per_round = (r->gstate[1].sorts - r->gstate[0].sorts);
+                r->gphase = 0;
+                s->graph_captures += 2;
+            } else {
+                destroy_graphs(r);
+                r->graphs_tried = true;
+            }
+        }
+    }
+    if (r->graph[0] && r->graph[1] && h->steps_since_sort == 0) {
+        const int k = r->gphase;
+        if (hipGraphLaunch(r->graph[k], h->stream) != hipSuccess) return PM_EDEVICE;
+        const int64_t sorts = h->sorts;
+        pmi::restore_state(h, r->gstate[k ^ 1]);
+        h->sorts = sorts + r->sorts_per_round;
+        r->gphase ^= 1;
+        s->graph_launches += 1;
+        return PM_OK;
+    }
+    return pm_step(h, steps);
+}
+
+void complete(pm_server* s, Request* r) {
+    pm_sim* h = r->sim;
+    r->x.resize((size_t)r->n);
+    r->y.resize((size_t)r->n);
+    r->vx.resize((size_t)r->n);
+    r->vy.resize((size_t)r->n);
+    const pm_status st = pm_read(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data());
+    destroy_graphs(r);
+    if (st != PM_OK) {
+        fail_request(s, r, h->err);
+        return;
+    }
+    r->sim = nullptr;
+    s->pool.push_back(h);
+    r->m.clear();
+    r->m.shrink_to_fit();
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->state = PM_REQ_DONE;
+    r->t_done = now_s();
+    s->completed += 1;
+    s->t_last_done = r->t_done;
+    s->req_lat_ms.push_back((float)((r->t_done - r->t_submit) * 1e3));
+    s->cv_done.notify_all();
+}
+
+void run_round(pm_server* s) {
+    const int R = s->cfg.mode == PM_SERVE_PLAIN ? 0 : std::max(2, s->cfg.round_steps & ~1);
+    std::vector<Request*> batched, large, tail;
+    std::vector<int> advance(s->active.size(), 0);
+    for (size_t i = 0; i < s->active.size(); ++i) {
+        Request* r = s->active[i];
+        const int rem = r->steps - r->done_steps;
+        if (R == 0) { advance[i] = rem; tail.push_back(r); continue; }
+        advance[i] = std::min(rem, R);
+        if (rem < R) tail.push_back(r);
+        else if (r->batched) batched.push_back(r);
+        else large.push_back(r);
+    }
+    const double t0 = now_s();
+    std::vector<std::pair<Request*, pm_status>> errs;
+    if (!batched.empty()) {
+        s->batch_err = "batched launch failed";
+        const pm_status st = run_batched_round(s, batched, R);
+        if (st != PM_OK)
+            for (Request* r : batched) errs.push_back({r, st});
+    }
+    for (Request* r : large) {
+        const pm_status st = run_large_round(s, r, R);
+        if (st != PM_OK) errs.push_back({r, st});
+    }
+    for (Request* r : tail) {
+        const pm_status st = pm_step(r->sim, r->steps - r->done_steps);
+        if (st != PM_OK) errs.push_back({r, st});
+    }
+    if (!batched.empty() && hipStreamSynchronize(s->batch_stream) != hipSuccess)
+        for (Request* r : batched) errs.push_back({r, PM_EDEVICE});
+    for (Request* r : s->active) {
+        if (hipStreamSynchronize(r->sim->stream) != hipSuccess) errs.push_back({r, PM_EDEVICE});
+    }
+    const double t1 = now_s();
+    s->rounds += 1;
+    std::vector<Request*> keep;
+    for (size_t i = 0; i < s->active.size(); ++i) {
+        Request* r = s->active[i];
+        const bool bad = std::any_of(errs.begin(), errs.end(), [&](const auto& e) { return e.first == r; });
+        if (bad) {
+            if (r->batched) s->plan.signature.clear();
+            fail_request(s, r, r->batched ? s->batch_err : r->sim->err);
+            continue;
+        }
+        const int k = advance[i];
+        r->done_steps += k;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            const float lat = k > 0 ? (float)((t1 - t0) * 1e3 / k) : 0.f;
+            for (int q = 0; q < k; ++q) s->step_lat_ms.push_back(lat);
+            s->particle_steps += (double)r->n * k;
+            s->steps_done += k;
+        }
+        if (r->done_steps >= r->steps) {
+            if (r->batched) s->plan.signature.clear();
+            complete(s, r);
+        } else {
+            keep.push_back(r);
+        }
+    }
+    s->active.swap(keep);
+}
+
+void worker_main(pm_server* s) {
+    (void)hipSetDevice(s->cfg.device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->active.empty(); });
+            if (s->stop && s->queue.empty() && s->active.empty()) break;
+        }
+        admit(s);
+        if (!s->active.empty()) run_round(s);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void pm_server_config_default(pm_server_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->hbm_budget_bytes = (int64_t)200 << 30;
+    c->max_active = 256;
+    c->mode = PM_SERVE_BATCHED;
+    c->round_steps = 8;
+    c->use_graphs = 1;
+    c->batch_max_lattice = 512;
+    c->device = 0;
+}
+
+int32_t pm_plan_admission(const int64_t* need_bytes, int32_t n_queued, int64_t free_bytes, int32_t max_new) {
+    int32_t k = 0;
+    while (k < n_queued && k < max_new && need_bytes && need_bytes[k] >= 0 && need_bytes[k] <= free_bytes) {
+        free_bytes -= need_bytes[k];
+        ++k;
+    }
+    return k;
+}
+
+pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
+    if (!cfg || !out) return PM_EINVAL;
+    *out = nullptr;
+    if (cfg->hbm_budget_bytes <= 0 || cfg->max_active <= 0 || cfg->round_steps <= 0 ||
+        (cfg->mode != PM_SERVE_PLAIN && cfg->mode != PM_SERVE_BATCHED))
+        return PM_EINVAL;
+    if (pm_device_count() <= cfg->device || cfg->device < 0) return PM_ENODEV;
+    pm_server* s = new (std::nothrow) pm_server();
+    if (!s) return PM_ENOMEM;
+    s->cfg = *cfg;
+    if (hipSetDevice(cfg->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->batch_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return PM_EDEVICE;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return PM_OK;
+}
+
+pm_status pm_server_submit(pm_server_handle s, const pm_config* cfg, const float* x, const float* y,
+                           const float* vx, const float* vy, const float* m, int64_t n, int32_t steps,
+                           int64_t* request_id) {
+    if (!s || !cfg || n < 0 || steps < 0 || (n > 0 && (!x || !y || !vx || !vy || !m))) return PM_EINVAL;
+    pm_config c = *cfg;
+    c.n_particles = n;
+    c.device = s->cfg.device;
+    std::string why;
+    if (pmi::validate(&c, &why) != PM_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = why;
+        return PM_EINVAL;
+    }
+    if (sim_bytes(c) > s->cfg.hbm_budget_bytes) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "request needs more device memory than the server budget";
+        return PM_ENOMEM;
+    }
+    auto r = std::make_unique<Request>();
+    r->cfg = c;
+    r->n = n;
+    r->steps = steps;
+    r->x.assign(x, x + n);
+    r->y.assign(y, y + n);
+    r->vx.assign(vx, vx + n);
+    r->vy.assign(vy, vy + n);
+    r->m.assign(m, m + n);
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = s->next_id++;
+    r->t_submit = now_s();
+    if (s->t_first_submit < 0) s->t_first_submit = r->t_submit;
+    if (request_id) *request_id = r->id;
+    s->queue.push_back(r.get());
+    s->reqs[r->id] = std::move(r);
+    s->submitted += 1;
+    s->cv_work.notify_all();
+    return PM_OK;
+}
+
+pm_status pm_server_status(pm_server_handle s, int64_t id, int32_t* state) {
+    if (!s || !state) return PM_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    *state = it == s->reqs.end() ? PM_REQ_UNKNOWN : it->second->state;
+    return PM_OK;
+}
+
+pm_status pm_server_wait(pm_server_handle s, int64_t id, float* x, float* y, float* vx, float* vy,
+                         double timeout_s) {
+    if (!s) return PM_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return PM_EINVAL;
+    Request* r = it->second.get();
+    auto done = [&] { return r->state == PM_REQ_DONE || r->state == PM_REQ_FAILED; };
+    if (timeout_s > 0) {
+        if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return PM_EBUSY;
+    } else {
+        s->cv_done.wait(lk, done);
+    }
+    if (r->state == PM_REQ_FAILED) {
+        s->err = r->msg;
+        s->reqs.erase(it);
+        return PM_EDEVICE;
+    }
+    const size_t b = (size_t)r->n * sizeof(float);
+    if (x) std::memcpy(x, r->x.data(), b);
+    if (y) std::memcpy(y, r->y.data(), b);
+    if (vx) std::memcpy(vx, r->vx.data(), b);
+    if (vy) std::memcpy(vy, r->vy.data(), b);
+    s->reqs.erase(it);
+    return PM_OK;
+}
+
+pm_status pm_server_drain(pm_server_handle s) {
+    if (!s) return PM_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->completed + s->failed >= s->submitted; });
+    return PM_OK;
+}
+
+pm_status pm_server_get_stats(pm_server_handle s, pm_server_stats* o) {
+    if (!s || !o) return PM_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(o, 0, sizeof(*o));
+    o->submitted = s->submitted;
+    o->completed = s->completed;
+    o->failed = s->failed;
+    o->active = (int64_t)s->active.size();
+    o->queued = (int64_t)s->queue.size();
+    o->particle_steps = s->particle_steps;
+    o->steps = s->steps_done;
+    o->first_submit_s = s->t_first_submit;
+    o->last_done_s = s->t_last_done;
+    o->step_latency_p50_ms = percentile(s->step_lat_ms, 0.50);
+    o->step_latency_p99_ms = percentile(s->step_lat_ms, 0.99);
+    o->request_latency_p50_ms = percentile(s->req_lat_ms, 0.50);
+    o->request_latency_p99_ms = percentile(s->req_lat_ms, 0.99);
+    o->bytes_in_use = s->bytes_in_use;
+    o->peak_bytes = s->peak_bytes;
+    o->rounds = s->rounds;
+    o->batched_launch_rounds = s->batched_rounds;
+    o->graph_launches = s->graph_launches;
+    o->graph_captures = s->graph_captures;
+    o->pool_hits = s->pool_hits;
+    return PM_OK;
+}
+
+const char* pm_server_last_error(pm_server_handle s) {
+    if (!s) return "null server";
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->err_copy = s->err;
+    return s->err_copy.c_str();
+}
+
+pm_status pm_server_destroy(pm_server_handle s) {
+    if (!s) return PM_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->cfg.device);
+    for (Request* r : s->active)
+        if (r->sim) { destroy_graphs(r); pm_destroy(r->sim); }
+    for (pm_sim* h : s->pool) pm_destroy(h);
+    if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
+    if (s->dscratch) (void)hipFree(s->dscratch);
+    if (s->hscratch) (void)hipHostFree(s->hscratch);
+    if (s->batch_stream) (void)hipStreamDestroy(s->batch_stream);
+    delete s;
+    return PM_OK;
+}
+
+}  // extern "C"

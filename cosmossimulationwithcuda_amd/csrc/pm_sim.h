@@ -1,0 +1,93 @@
+// Internal state of one simulation handle and the host-side helpers shared by
+// the C-ABI layer (pm_api.cpp) and the serving engine (pm_serve.cpp).  Not
+// installed; the public interface is include/pm.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pm.h"
+#include "pm_internal.h"
+
+namespace pmi {
+
+struct ProfRec {
+    int kernel;
+    hipEvent_t start, end;
+};
+
+}  // namespace pmi
+
+struct pm_sim {
+    pm_config cfg{};
+    hipStream_t stream = nullptr;
+    int64_t cap = 0, np = 0;
+    int shift = 40;
+    pmk::ParticleArrays cur{}, alt{};      // sorted state and the counting-sort target / read scratch
+    pmk::BinState bins{};
+    int nbins = 0;                         // in-bounds bins + parked bin
+    int nbins_alloc = 0;                   // bins the bin arrays were sized for
+    int max_chunks = 0;
+    int resort_interval = 8;
+    int steps_since_sort = 0;
+    int64_t sorts = 0;
+    unsigned long long* migrants = nullptr;
+    int64_t* rho_buf[2] = {nullptr, nullptr};
+    int rho_cur = 0;                       // rho_buf[rho_cur]: mass lattice of the current positions (when rho_valid)
+    int field_parity = -1;                 // rho_buf index the last field solve consumed (potential energy)
+    pmk::FieldDesc* fdesc = nullptr;       // device descriptors, [p] reads rho_buf[p]
+    pmk::FftPlan plan{};
+    bool rho_valid = false;
+    float2 *H = nullptr, *Hfy = nullptr;
+    float* T = nullptr;
+    float* dk = nullptr;
+    float2* tw = nullptr;
+    float* phi = nullptr;
+    float2* force = nullptr;
+    double* diag = nullptr;
+    int64_t bytes = 0;
+    std::vector<void*> allocs;
+    std::string err;
+    bool profiling = false;
+    std::vector<pmi::ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[PM_K_COUNT] = {};
+    int64_t prof_n[PM_K_COUNT] = {};
+};
+
+namespace pmi {
+
+pm_status fail(pm_sim* h, pm_status st, const std::string& msg);
+pm_status hip_fail(pm_sim* h, hipError_t e, const char* what);
+
+#define PM_HIP(h, call)                                        \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return pmi::hip_fail((h), e_, #call); \
+    } while (0)
+
+pm_status validate(const pm_config* c, std::string* why);
+int64_t align256(int64_t b);
+pm_status upload_field_desc(pm_sim* h);
+pm_status sort_now(pm_sim* h, bool counted);
+pm_status stage_deposit(pm_sim* h);
+pm_status stage_field(pm_sim* h);
+pm_status stage_push(pm_sim* h);
+pm_status stage_push_deposit(pm_sim* h);
+// Re-targets an existing handle (same lattice size, capacity >= its needs) to
+// another configuration: rebuilds the transfer function and the stencil symbol
+// when the physics parameters differ.
+pm_status reconfigure(pm_sim* h, const pm_config* cfg);
+
+// Host-side state that pm_step advances (for replaying captured step graphs).
+struct HostState {
+    pmk::ParticleArrays cur, alt;
+    int rho_cur, field_parity, steps_since_sort;
+    bool rho_valid;
+    int64_t sorts;
+};
+HostState save_state(const pm_sim* h);
+void restore_state(pm_sim* h, const HostState& s);
+
+}  // namespace pmi

@@ -16,6 +16,14 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(uint4* __restrict__ p, siz
     if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
 
+// blockIdx.y selects the buffer; every buffer is a multiple of 16 bytes.
+__global__ __launch_bounds__(kBlock) void k_fill_zero_batched(const FillDesc* __restrict__ d) {
+    const FillDesc f = d[blockIdx.y];
+    uint4* p = reinterpret_cast<uint4*>(f.p);
+    const size_t n16 = f.bytes / 16, stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -94,6 +102,15 @@ hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(kBlock), 0, s, reinterpret_cast<uint4*>(p), n16,
                        reinterpret_cast<unsigned char*>(p) + n16 * 16, ntail);
+    return hipGetLastError();
+}
+
+hipError_t fill_zero_batched(const FillDesc* d, int count, size_t max_bytes, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    size_t blocks = (max_bytes / 16 + kBlock - 1) / kBlock;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_fill_zero_batched, dim3((unsigned)blocks, count), dim3(kBlock), 0, s, d);
     return hipGetLastError();
 }
 

@@ -105,6 +105,7 @@ def lib():
         L.pm_last_error.restype = ctypes.c_char_p
         L.pm_status_string.argtypes = [ctypes.c_int]
         L.pm_status_string.restype = ctypes.c_char_p
+        _declare_server(L)
         _lib = L
     return _lib
 
@@ -258,3 +259,130 @@ class Simulation:
             _check(lib().pm_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(cnt)), self._h)
             res[name] = (ms.value, cnt.value)
         return res
+
+
+# ----- serving engine (include/pm_serve.h) -----------------------------------------
+
+SERVE_PLAIN, SERVE_BATCHED = 0, 1
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED, REQ_UNKNOWN = range(5)
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_active", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("round_steps", ctypes.c_int32),
+        ("use_graphs", ctypes.c_int32),
+        ("batch_max_lattice", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "active", "queued")] + [
+        (k, ctypes.c_double) for k in ("particle_steps", "steps", "first_submit_s", "last_done_s",
+                                       "step_latency_p50_ms", "step_latency_p99_ms",
+                                       "request_latency_p50_ms", "request_latency_p99_ms")] + [
+        (k, ctypes.c_int64) for k in ("bytes_in_use", "peak_bytes", "rounds", "batched_launch_rounds",
+                                      "graph_launches", "graph_captures", "pool_hits")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _declare_server(L):
+    L.pm_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.pm_plan_admission.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
+    L.pm_plan_admission.restype = ctypes.c_int32
+    L.pm_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_H)]
+    L.pm_server_submit.argtypes = [_H, ctypes.POINTER(Config), _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int64,
+                                   ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+    L.pm_server_status.argtypes = [_H, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
+    L.pm_server_wait.argtypes = [_H, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_double]
+    L.pm_server_drain.argtypes = [_H]
+    L.pm_server_get_stats.argtypes = [_H, ctypes.POINTER(ServerStats)]
+    L.pm_server_destroy.argtypes = [_H]
+    L.pm_server_last_error.argtypes = [_H]
+    L.pm_server_last_error.restype = ctypes.c_char_p
+
+
+def default_server_config(**kw):
+    c = ServerConfig()
+    lib().pm_server_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+def plan_admission(need_bytes, free_bytes, max_new):
+    """Number of queue-head requests admitted (strict FIFO), pure host logic."""
+    a = (ctypes.c_int64 * max(1, len(need_bytes)))(*need_bytes)
+    return lib().pm_plan_admission(a, len(need_bytes), int(free_bytes), int(max_new))
+
+
+class Server:
+    """Many independent simulations served concurrently on one device."""
+
+    def __init__(self, cfg=None, **kw):
+        self.cfg = cfg if cfg is not None else default_server_config(**kw)
+        h = _H()
+        st = lib().pm_server_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if st != PM_OK:
+            raise PMError(st, lib().pm_last_error(None).decode(errors="replace") or "pm_server_create failed")
+        self._h = h
+        self._n = {}
+
+    def _check(self, st):
+        if st != PM_OK:
+            raise PMError(st, lib().pm_server_last_error(self._h).decode(errors="replace"))
+
+    def submit(self, cfg, x, y, vx, vy, m, steps):
+        arrs = [_f32(a) for a in (x, y, vx, vy, m)]
+        n = arrs[0].size
+        rid = ctypes.c_int64()
+        self._check(lib().pm_server_submit(self._h, ctypes.byref(cfg), *arrs, n, int(steps), ctypes.byref(rid)))
+        self._n[rid.value] = n
+        return rid.value
+
+    def status(self, rid):
+        st = ctypes.c_int32()
+        self._check(lib().pm_server_status(self._h, rid, ctypes.byref(st)))
+        return st.value
+
+    def wait(self, rid, timeout_s=0.0):
+        """Final (x, y, vx, vy) of a request, in load order."""
+        n = self._n[rid]
+        out = [np.empty(n, np.float32) for _ in range(4)]
+        self._check(lib().pm_server_wait(self._h, rid, *[o.ctypes.data for o in out], float(timeout_s)))
+        del self._n[rid]
+        return tuple(out)
+
+    def drain(self):
+        self._check(lib().pm_server_drain(self._h))
+
+    def stats(self):
+        s = ServerStats()
+        self._check(lib().pm_server_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pm_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,123 @@
+"""GPU tests of the serving engine (include/pm_serve.h) through the C ABI.
+
+Bar: every request's final state is bit-identical to the same simulation run
+alone through pm_create/pm_step, whatever the serving mode, the other requests
+sharing its rounds, batched launches, captured-graph replays or pooled
+(reused) simulation memory.
+"""
+import numpy as np
+import pytest
+
+from cosmossimulationwithcuda_amd import ics, pm
+
+pytestmark = pytest.mark.gpu
+
+
+def request(n, npart, mode, seed, **kw):
+    cfg = dict(n_lattice=n, n_particles=npart, mode=mode, dt=0.02, gravity=1.0, green_origin=1.0,
+               deconv_eps=1e-3)
+    cfg.update(kw)
+    x, y, vx, vy, m = ics.exponential_disk(npart, n / 2, n / 2, n / 12, n / 3, seed=seed)
+    return pm.default_config(**cfg), (x, y, vx, vy, m)
+
+
+def run_alone(cfg, arrays, steps):
+    with pm.Simulation(cfg) as s:
+        s.load(*arrays)
+        s.step(steps)
+        return s.read()
+
+
+MIX = [  # (lattice, particles, mode, steps, extra config)
+    (256, 20000, pm.CIC, 37, {}),
+    (256, 5000, pm.NGP, 24, {"wrap_mode": pm.WRAP_REFERENCE}),
+    (512, 60000, pm.CIC, 53, {}),
+    (1024, 200000, pm.CIC, 53, {}),
+    (1024, 150000, pm.NGP, 16, {}),
+    (512, 30000, pm.CIC, 21, {"force_mode": pm.FORCE_STENCIL}),
+]
+
+
+@pytest.fixture(scope="module")
+def mix():
+    reqs = [request(n, k, mode, seed=10 + i, **kw) for i, (n, k, mode, _, kw) in enumerate(MIX)]
+    want = [run_alone(c, a, st) for (c, a), (_, _, _, st, _) in zip(reqs, MIX)]
+    return reqs, want
+
+
+@pytest.mark.parametrize("mode,graphs,batch_max", [
+    (pm.SERVE_BATCHED, 1, 512),
+    (pm.SERVE_BATCHED, 0, 512),
+    (pm.SERVE_BATCHED, 1, 0),       # no batched launches: every request on its own stream
+    (pm.SERVE_PLAIN, 0, 512),
+])
+def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max):
+    reqs, want = mix
+    print(f"serving mode={mode} graphs={graphs} batch_max_lattice={batch_max}", flush=True)
+    with pm.Server(mode=mode, use_graphs=graphs, batch_max_lattice=batch_max,
+                   hbm_budget_bytes=8 << 30) as srv:
+        ids = [srv.submit(c, *a, steps=st) for (c, a), (_, _, _, st, _) in zip(reqs, MIX)]
+        got = [srv.wait(i, timeout_s=120) for i in ids]
+        stats = srv.stats()
+    for k, (g, w) in enumerate(zip(got, want)):
+        for a, b in zip(g, w):
+            assert np.array_equal(a, b, equal_nan=True), f"request {k} {MIX[k][:4]} differs"
+    assert stats["completed"] == len(MIX) and stats["failed"] == 0
+    assert stats["particle_steps"] == sum(k * st for _, k, _, st, _ in MIX)
+    if mode == pm.SERVE_BATCHED and batch_max:
+        assert stats["batched_launch_rounds"] > 0
+    if mode == pm.SERVE_BATCHED and graphs:
+        assert stats["graph_launches"] >= 4 and stats["graph_captures"] >= 2
+    if mode == pm.SERVE_PLAIN:
+        assert stats["rounds"] == len(MIX)
+
+
+def test_graph_replays_interleaved_with_eager_work():
+    """A large simulation replayed from its two captured round graphs while
+    other requests run eager steps and get admitted between the replays: the
+    state after every replayed round equals the eager run."""
+    n, k = 1024, 100000
+    cfg, arrays = request(n, k, pm.CIC, seed=3)
+    checkpoints = [16, 24, 32, 40, 48]
+    want = {st: run_alone(cfg, arrays, st) for st in checkpoints}
+    with pm.Server(use_graphs=1, batch_max_lattice=256, hbm_budget_bytes=8 << 30) as srv:
+        for st in checkpoints:
+            scfg, sarr = request(256, 3000, pm.NGP, seed=st)
+            side = [srv.submit(scfg, *sarr, steps=5)]
+            rid = srv.submit(cfg, *arrays, steps=st)
+            got = srv.wait(rid, timeout_s=120)
+            for sid in side:
+                srv.wait(sid, timeout_s=120)
+            for a, b in zip(got, want[st]):
+                assert np.array_equal(a, b, equal_nan=True), f"after {st} steps"
+        stats = srv.stats()
+    assert stats["graph_launches"] >= 6 and stats["pool_hits"] >= 4
+
+
+def test_budget_admission_and_pooling():
+    cfg, arrays = request(1024, 100000, pm.CIC, seed=5)
+    one = pm.instance_bytes(pm.default_config(n_lattice=1024, n_particles=131072))
+    budget = int(one * 1.5)                 # one simulation at a time
+    with pm.Server(hbm_budget_bytes=budget) as srv:
+        ids = [srv.submit(cfg, *arrays, steps=8) for _ in range(3)]
+        outs = [srv.wait(i, timeout_s=120) for i in ids]
+        stats = srv.stats()
+        big = pm.default_config(n_lattice=2048, n_particles=1_000_000)
+        with pytest.raises(pm.PMError) as e:
+            srv.submit(big, *request(2048, 1000, pm.CIC, seed=6)[1], steps=1)
+        assert e.value.status == pm.PM_ENOMEM
+    want = run_alone(cfg, arrays, 8)
+    for o in outs:
+        for a, b in zip(o, want):
+            assert np.array_equal(a, b, equal_nan=True)
+    assert stats["peak_bytes"] <= budget
+    assert stats["pool_hits"] == 2
+
+
+def test_invalid_request_rejected_at_submit():
+    with pm.Server() as srv:
+        bad = pm.default_config(n_lattice=100, n_particles=10)
+        z = np.zeros(10, np.float32)
+        with pytest.raises(pm.PMError) as e:
+            srv.submit(bad, z, z, z, z, z, steps=1)
+        assert e.value.status == pm.PM_EINVAL and "power of two" in str(e.value)

@@ -66,3 +66,48 @@ def test_null_handle_calls_return_einval():
     assert L.pm_sync(None) == pm.PM_EINVAL
     assert L.pm_destroy(None) == pm.PM_EINVAL
     assert L.pm_status_string(pm.PM_ENOMEM) == b"out of device memory"
+
+
+def test_library_exports_every_serving_symbol():
+    lib = pm.lib()
+    names = declared_functions("pm_serve.h")
+    assert "pm_server_submit" in names and "pm_plan_admission" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_server_config_defaults():
+    c = pm.default_server_config()
+    assert c.mode == pm.SERVE_BATCHED and c.round_steps == 8 and c.use_graphs == 1
+    assert c.hbm_budget_bytes == 200 << 30 and c.max_active > 1
+
+
+def test_plan_admission_is_strict_fifo():
+    gib = 1 << 30
+    assert pm.plan_admission([], 10 * gib, 4) == 0
+    assert pm.plan_admission([gib, gib, gib], 10 * gib, 8) == 3
+    assert pm.plan_admission([gib, gib, gib], 10 * gib, 2) == 2          # slot limit
+    assert pm.plan_admission([4 * gib, 4 * gib, 4 * gib], 10 * gib, 8) == 2
+    # the head that does not fit blocks smaller requests behind it (no overtaking)
+    assert pm.plan_admission([11 * gib, gib], 10 * gib, 8) == 0
+    assert pm.plan_admission([6 * gib, 5 * gib, gib], 10 * gib, 8) == 1
+    assert pm.plan_admission([10 * gib], 10 * gib, 1) == 1                # exact fit
+    assert pm.plan_admission([gib], 0, 1) == 0
+    assert pm.plan_admission([-1, gib], 10 * gib, 8) == 0                 # invalid size stops
+
+
+def test_server_create_validates_before_device():
+    h = ctypes.c_void_p()
+    for kw in ({"hbm_budget_bytes": 0}, {"max_active": 0}, {"round_steps": 0}, {"mode": 5}):
+        c = pm.default_server_config(**kw)
+        assert pm.lib().pm_server_create(ctypes.byref(c), ctypes.byref(h)) == pm.PM_EINVAL
+    assert pm.lib().pm_server_destroy(None) == pm.PM_EINVAL
+
+
+@pytest.mark.skipif(pm.device_count() > 0, reason="checks the no-device path")
+def test_server_without_device_reports_enodev():
+    h = ctypes.c_void_p()
+    c = pm.default_server_config()
+    assert pm.lib().pm_server_create(ctypes.byref(c), ctypes.byref(h)) == pm.PM_ENODEV
+    with pytest.raises(pm.PMError):
+        pm.Server(c)

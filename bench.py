@@ -102,6 +102,105 @@ def copy_probe(nbytes):
     print(f"# copy probe: {nbytes} bytes device to device", file=sys.stderr, flush=True)
 
 
+# Serving ensemble: independent requests arriving as a Poisson stream.
+SERVING = {"requests": 512, "rate_per_s": 40.0, "lattices": (256, 512, 1024, 2048), "particles": (1e4, 1e7),
+           "steps": 200, "budget_bytes": 200 << 30, "dt": 1.0, "template": 10_000_000}
+
+
+def serving_workload(nreq, seed):
+    """Seeded request list: lattice uniform over the sizes, particle count
+    log-uniform, disk kind (exponential / uniform) uniform, Poisson arrivals."""
+    rng = np.random.default_rng(seed)
+    lo, hi = SERVING["particles"]
+    lat = rng.choice(SERVING["lattices"], nreq)
+    npart = np.exp(rng.uniform(np.log(lo), np.log(hi), nreq)).astype(np.int64)
+    kind = rng.integers(0, 2, nreq)
+    arrive = np.cumsum(rng.exponential(1.0 / SERVING["rate_per_s"], nreq))
+    seeds = rng.integers(0, 2 ** 31, nreq)
+    return [dict(n_lattice=int(a), n=int(b), kind=int(c), t=float(d), seed=int(e))
+            for a, b, c, d, e in zip(lat, npart, kind, arrive, seeds)]
+
+
+def serving_run(pm, ics, reqs, templates, mode, poisson):
+    """Submits the requests (at their arrival times, or all at once), waits for
+    every result, returns the server statistics."""
+    import queue
+    import threading
+    srv = pm.Server(pm.default_server_config(mode=mode, hbm_budget_bytes=SERVING["budget_bytes"]))
+    q = queue.Queue()
+    errors = []
+
+    def submitter():
+        try:
+            t0 = time.perf_counter()
+            for r in reqs:
+                if poisson:
+                    d = t0 + r["t"] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
+                cfg = pm.default_config(n_lattice=r["n_lattice"], n_particles=r["n"], mode=pm.CIC,
+                                        dt=SERVING["dt"], gravity=g, green_origin=1.0, deconv_eps=1e-3)
+                q.put(srv.submit(cfg, x, y, vx, vy, m, SERVING["steps"]))
+        except Exception as e:          # reported by the waiting side
+            errors.append(e)
+        q.put(None)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    done = 0
+    while True:
+        rid = q.get()
+        if rid is None:
+            break
+        srv.wait(rid, copy=False)
+        done += 1
+        if done % 128 == 0:
+            print(f"#   {done} requests done", file=sys.stderr, flush=True)
+    th.join()
+    st = srv.stats()
+    srv.close()
+    if errors:
+        raise errors[0]
+    wall = st["last_done_s"] - st["first_submit_s"]
+    return {"wall_s": round(wall, 3),
+            "steps_per_s": st["steps"] / wall,
+            "particle_steps_per_s": st["particle_steps"] / wall,
+            "step_latency_p50_ms": round(st["step_latency_p50_ms"], 4),
+            "step_latency_p99_ms": round(st["step_latency_p99_ms"], 4),
+            "request_latency_p50_ms": round(st["request_latency_p50_ms"], 2),
+            "request_latency_p99_ms": round(st["request_latency_p99_ms"], 2),
+            "peak_GB": round(st["peak_bytes"] / 1e9, 2), "rounds": st["rounds"],
+            "graph_launches": st["graph_launches"], "batched_rounds": st["batched_launch_rounds"],
+            "pool_hits": st["pool_hits"], "graph_captures": st["graph_captures"], "failed": st["failed"],
+            "worker_s": {k: round(st[k], 3) for k in ("admit_s", "launch_s", "readback_s", "wait_s")}}
+
+
+def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson")):
+    from cosmossimulationwithcuda_amd import ics, pm
+    t = time.perf_counter()
+    templates = [ics.disk_template(k, SERVING["template"], seed + i) for i, k in enumerate(("exponential", "uniform"))]
+    reqs = serving_workload(nreq, seed)
+    print(f"# serving: {nreq} requests, {sum(r['n'] for r in reqs)} particles total, templates in "
+          f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    out = {"workload": f"ensemble_{nreq}req_poisson{SERVING['rate_per_s']:g}_{SERVING['steps']}steps_cic",
+           "requests": nreq, "steps_per_request": SERVING["steps"],
+           "lattices": list(SERVING["lattices"]), "particles_loguniform": list(SERVING["particles"]),
+           "budget_GB": SERVING["budget_bytes"] / 2 ** 30}
+    for tag, poisson in (("saturated", False), ("poisson", True)):
+        if tag not in arrivals:
+            continue
+        for name, mode in (("plain", pm.SERVE_PLAIN), ("batched", pm.SERVE_BATCHED)):
+            if name not in modes:
+                continue
+            print(f"# serving {tag} {name}", file=sys.stderr, flush=True)
+            out[f"{tag}_{name}"] = serving_run(pm, ics, reqs, templates, mode, poisson)
+        if len(modes) == 2:
+            p, b = out[f"{tag}_plain"], out[f"{tag}_batched"]
+            out[f"{tag}_speedup_steps_per_s"] = round(b["steps_per_s"] / p["steps_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +213,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (hipMemcpy) as the calibration "
                          "kernel of known size for rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--serving", choices=("on", "off", "only"), default="on",
+                    help="serving ensemble leg (plain vs batched server, saturated and Poisson arrivals)")
+    ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
+    ap.add_argument("--serving-modes", default="plain,batched")
+    ap.add_argument("--serving-arrivals", default="saturated,poisson")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -122,6 +226,11 @@ def main():
         sys.exit(2)
 
     from cosmossimulationwithcuda_amd import pm
+
+    if args.serving == "only":
+        print(json.dumps({"serving": serving_bench(args.serving_requests, args.seed, args.serving_modes.split(","),
+                                   args.serving_arrivals.split(","))}), flush=True)
+        return
 
     n, np_, mode, dt, gen = WORKLOADS[args.workload]
     t_ic = time.perf_counter()
@@ -203,6 +312,9 @@ def main():
                                          f"(int64 deposit, FP64 FFT field, FP32 push), host {cpu_model()}"}
     else:
         out["cpu_baseline"] = None
+    if args.serving == "on":
+        out["serving"] = serving_bench(args.serving_requests, args.seed, args.serving_modes.split(","),
+                                   args.serving_arrivals.split(","))
     print(json.dumps(out), flush=True)
 
 

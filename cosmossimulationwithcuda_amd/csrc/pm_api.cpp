@@ -93,11 +93,13 @@ pm_status dalloc(pm_sim* h, void** p, int64_t bytes) {
 
 void release(pm_sim* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->own_stream && h->own_stream != h->stream) (void)hipStreamSynchronize(h->own_stream);
     for (auto& r : h->recs) { (void)hipEventDestroy(r.start); (void)hipEventDestroy(r.end); }
     for (auto e : h->pool) (void)hipEventDestroy(e);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    h->own_stream = nullptr;
     h->stream = nullptr;
 }
 
@@ -295,20 +297,27 @@ pm_status upload_tables(pm_sim* h) {
     return PM_OK;
 }
 
-pm_status build_spectra(pm_sim* h) {
+// The FP64 setup work (2 N^2 doubles) lives in the two mass lattices, which
+// are contiguous and cleared before every deposit: no allocation, so no
+// device-wide synchronisation of a free.  With T_src (a transfer function
+// built for the same lattice, mode, G(0) and eps) the spectra are copied.
+pm_status build_spectra(pm_sim* h, const float* T_src) {
     const int n = h->cfg.n_lattice;
-    double* work = nullptr;
-    hipError_t e = hipMalloc(&work, (size_t)16 * n * n);
-    if (e != hipSuccess) return hip_fail(h, e, "hipMalloc(setup work)");
-    e = pmk::build_transfer(n, h->cfg.mode, h->cfg.green_origin, h->cfg.deconv_eps, work, h->T, h->stream);
+    hipError_t e;
+    if (T_src) {
+        e = hipMemcpyAsync(h->T, T_src, (size_t)4 * (n / 2 + 1) * n, hipMemcpyDeviceToDevice, h->stream);
+    } else {
+        double* work = reinterpret_cast<double*>(h->rho_buf[0]);
+        e = pmk::build_transfer(n, h->cfg.mode, h->cfg.green_origin, h->cfg.deconv_eps, work, h->T, h->stream);
+    }
     hipError_t e2 = hipStreamSynchronize(h->stream);
-    (void)hipFree(work);
+    h->rho_valid = false;
     if (e != hipSuccess) return hip_fail(h, e, "build_transfer");
     if (e2 != hipSuccess) return hip_fail(h, e2, "build_transfer sync");
     return PM_OK;
 }
 
-pm_status reconfigure(pm_sim* h, const pm_config* cfg) {
+pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src) {
     std::string why;
     if (validate(cfg, &why) != PM_OK) return fail(h, PM_EINVAL, why);
     if (cfg->n_lattice != h->cfg.n_lattice || cfg->n_particles > h->cap || cfg->device != h->cfg.device)
@@ -321,7 +330,7 @@ pm_status reconfigure(pm_sim* h, const pm_config* cfg) {
     h->cfg.n_particles = cap;
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
     pm_status st = PM_OK;
-    if (spectra && (st = build_spectra(h)) != PM_OK) return st;
+    if (spectra && (st = build_spectra(h, T_src)) != PM_OK) return st;
     if (tables && (st = upload_tables(h)) != PM_OK) return st;
     h->np = 0;
     h->rho_valid = false;
@@ -378,7 +387,11 @@ int pm_device_count(void) {
     return n;
 }
 
-pm_status pm_create(const pm_config* cfg, pm_handle* out) {
+}  // extern "C"
+
+namespace pmi {
+
+pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
     if (!out) return fail(nullptr, PM_EINVAL, "null output handle");
     *out = nullptr;
     std::string why;
@@ -396,10 +409,11 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     const int n = cfg->n_lattice, nk = n / 2 + 1;
     pm_status st = PM_OK;
     auto bail = [&](pm_status s) { release(h); std::string e = h->err; delete h; g_last_error = e; return s; };
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-        h->stream = nullptr;
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        h->own_stream = nullptr;
         return bail(fail(h, PM_EDEVICE, "hipStreamCreate failed"));
     }
+    h->stream = h->own_stream;
     void* p;
     float** parts[10] = {&h->cur.x, &h->cur.y, &h->cur.vx, &h->cur.vy, &h->cur.m,
                          &h->alt.x, &h->alt.y, &h->alt.vx, &h->alt.vy, &h->alt.m};
@@ -425,8 +439,10 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
         h->bins.start = reinterpret_cast<int64_t*>(bp + 3 * ib);
         h->migrants = reinterpret_cast<unsigned long long*>(bp + 3 * ib + align256(8 * (int64_t)(h->nbins + 2)));
     }
-    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_buf[0] = static_cast<int64_t*>(p);
-    if ((st = dalloc(h, &p, L.rho)) != PM_OK) return bail(st); h->rho_buf[1] = static_cast<int64_t*>(p);
+    // both mass lattices in one block: 2 * 8 N^2 bytes, also the FP64 setup work area
+    if ((st = dalloc(h, &p, 2 * L.rho)) != PM_OK) return bail(st);
+    h->rho_buf[0] = static_cast<int64_t*>(p);
+    h->rho_buf[1] = h->rho_buf[0] + L.rho / 8;
     if ((st = dalloc(h, &p, L.fdesc)) != PM_OK) return bail(st);
     h->fdesc = static_cast<pmk::FieldDesc*>(p);
     h->plan = pmk::default_plan(n);
@@ -440,7 +456,7 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     if ((st = dalloc(h, &p, L.diag)) != PM_OK) return bail(st); h->diag = static_cast<double*>(p);
 
     if ((st = upload_tables(h)) != PM_OK) return bail(st);
-    if ((st = build_spectra(h)) != PM_OK) return bail(st);
+    if ((st = build_spectra(h, T_src)) != PM_OK) return bail(st);
     // deterministic initial lattices (force/potential read before any step are zero)
     if (pmk::fill_zero(h->phi, (size_t)4 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||
@@ -453,6 +469,12 @@ pm_status pm_create(const pm_config* cfg, pm_handle* out) {
     *out = h;
     return PM_OK;
 }
+
+}  // namespace pmi
+
+extern "C" {
+
+pm_status pm_create(const pm_config* cfg, pm_handle* out) { return pmi::create(cfg, out, nullptr); }
 
 pm_status pm_destroy(pm_handle h) {
     if (!h) return fail(nullptr, PM_EINVAL, "null handle");

@@ -1,9 +1,20 @@
 // Serving engine (include/pm_serve.h): continuous admission of independent
 // simulation requests under a device-memory budget, device-resident state,
 // rounds that advance every admitted request together, batched launches for
-// small lattices, captured HIP graphs for large ones, latency statistics.
+// small simulations, captured HIP graphs for large ones, latency statistics.
 // No counterpart in the reference (its host side runs one simulation per GPU,
 // CosmosCuda.cuh:1302-1395).
+//
+// Two host threads:
+//   io      admits queued requests (pooled or new simulation, upload, first
+//           deposit) and reads finished ones back, on its own stream;
+//   worker  runs rounds over the admitted requests: small simulations in
+//           batched launches on one stream, the others spread over a few
+//           lane streams (captured round graphs or eager steps).
+// Hand-offs go through `ready` (io -> worker) and `finished` (worker -> io),
+// so uploads and read-backs overlap the rounds.  Every simulation's
+// arithmetic is the same as alone: results do not depend on the mode or on
+// the other requests.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -15,6 +26,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pm_serve.h"
@@ -24,6 +36,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
+
+constexpr int kLanes = 2;   // streams for non-batched simulations (with the batch and io streams: 4 queues)
 
 int64_t capacity_class(int64_t n) {
     int64_t c = 1024;
@@ -43,7 +57,8 @@ struct Request {
     std::string msg;
     pm_sim* sim = nullptr;
     bool batched = false;
-    // captured rounds of a large simulation: graph[k] advances from state[k] to state[k ^ 1]
+    bool failed = false;    // set by the worker; the io thread reports it
+    // captured rounds of a large simulation: graph[k] advances from gstate[k] to gstate[k ^ 1]
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     pmi::HostState gstate[2];
     int64_t sorts_per_round = 0;
@@ -55,7 +70,6 @@ struct Request {
 void destroy_graphs(Request* r) {
     for (auto& g : r->graph)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-    r->graphs_tried = false;
 }
 
 template <typename T>
@@ -89,33 +103,44 @@ struct BatchPlan {
     hipGraphExec_t graph = nullptr;
 };
 
+using SpectraKey = std::tuple<int, int, double, double>;   // lattice, mode, G(0), eps
+
 }  // namespace
 
 struct pm_server {
     pm_server_config cfg{};
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request*> queue;
+    std::condition_variable cv_io, cv_work, cv_done;
+    // guarded by mu
+    std::deque<Request*> queue;                      // submitted, not admitted
     std::map<int64_t, std::unique_ptr<Request>> reqs;
-    std::vector<Request*> active;
-    std::vector<pm_sim*> pool;
-    int64_t bytes_in_use = 0, peak_bytes = 0;
-    std::thread worker;
+    std::vector<Request*> ready;                     // admitted (io -> worker)
+    std::vector<Request*> finished;                  // stepped to the end or failed (worker -> io)
+    int64_t slots_used = 0;                          // admitted and not yet read back
+    int64_t n_active = 0;
     bool stop = false;
     int64_t next_id = 1;
-    std::string err, err_copy;   // last failure reported to callers (guarded by mu)
-    std::string batch_err;       // worker only
-    // statistics (guarded by mu)
+    std::string err, err_copy;
+    int64_t bytes_in_use = 0, peak_bytes = 0;
     std::vector<float> step_lat_ms, req_lat_ms;
     double particle_steps = 0, steps_done = 0, t_first_submit = -1, t_last_done = 0;
     int64_t submitted = 0, completed = 0, failed = 0;
     int64_t rounds = 0, batched_rounds = 0, graph_launches = 0, graph_captures = 0, pool_hits = 0;
-    // batched launches
-    hipStream_t batch_stream = nullptr;
+    double t_admit = 0, t_launch = 0, t_readback = 0, t_wait = 0;
+    // io thread only
+    std::vector<pm_sim*> pool;
+    std::map<SpectraKey, float*> spectra;            // transfer functions by physics parameters
+    bool budget_blocked = false;
+    // worker only
+    std::vector<Request*> active;
+    std::string batch_err;
     void* dscratch = nullptr;
     size_t dscratch_bytes = 0;
     void* hscratch = nullptr;
     BatchPlan plan;
+    // streams (created once, default priority)
+    hipStream_t batch_stream = nullptr, io_stream = nullptr, lanes[kLanes] = {};
+    std::thread worker, io;
 };
 
 namespace {
@@ -126,12 +151,21 @@ int64_t sim_bytes(const pm_config& c) {
     return pm_instance_bytes(&k);
 }
 
-void fail_request(pm_server* s, Request* r, std::string msg) {   // by value: msg may live in r->sim
+// ---- io thread: admission and read-back --------------------------------------------
+
+void destroy_sim(pm_server* s, pm_sim* h) {
+    const int64_t b = h->bytes;
+    pm_destroy(h);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->bytes_in_use -= b;
+}
+
+void finish_failed(pm_server* s, Request* r, std::string msg) {   // by value: msg may live in r->sim
     if (r->sim) {
         destroy_graphs(r);
-        s->bytes_in_use -= r->sim->bytes;
-        pm_destroy(r->sim);
+        pm_sim* h = r->sim;
         r->sim = nullptr;
+        destroy_sim(s, h);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     r->state = PM_REQ_FAILED;
@@ -141,85 +175,208 @@ void fail_request(pm_server* s, Request* r, std::string msg) {   // by value: ms
     s->cv_done.notify_all();
 }
 
-// A pooled simulation of the same lattice size, device, and a capacity within
-// 2x of what the request needs; nullptr if none.
+// A pooled simulation of the same lattice whose capacity holds the request
+// and is at most 4x its capacity class (the smallest such); nullptr if none.
 pm_sim* take_pooled(pm_server* s, const Request* r) {
     const int64_t need = capacity_class(r->n);
+    int best = -1;
     for (size_t i = 0; i < s->pool.size(); ++i) {
-        pm_sim* h = s->pool[i];
-        if (h->cfg.n_lattice == r->cfg.n_lattice && h->cap >= r->n && h->cap <= 2 * need) {
-            s->pool.erase(s->pool.begin() + (long)i);
-            return h;
-        }
+        const pm_sim* h = s->pool[i];
+        if (h->cfg.n_lattice == r->cfg.n_lattice && h->cap >= r->n && h->cap <= 4 * need &&
+            (best < 0 || h->cap < s->pool[(size_t)best]->cap))
+            best = (int)i;
     }
-    return nullptr;
+    if (best < 0) return nullptr;
+    pm_sim* h = s->pool[(size_t)best];
+    s->pool.erase(s->pool.begin() + best);
+    return h;
 }
 
-// Admit from the queue head while memory and slots allow (strict FIFO).
-void admit(pm_server* s) {
-    const int max_active = s->cfg.mode == PM_SERVE_PLAIN ? 1 : std::max(1, s->cfg.max_active);
-    while ((int)s->active.size() < max_active) {
-        Request* r;
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            if (s->queue.empty()) return;
-            r = s->queue.front();
-        }
-        pm_sim* h = take_pooled(s, r);
-        if (h) {
-            s->pool_hits += 1;
-        } else {
-            const int64_t need = sim_bytes(r->cfg);
-            // make room by releasing idle pooled simulations, largest first
-            std::sort(s->pool.begin(), s->pool.end(), [](pm_sim* a, pm_sim* b) { return a->bytes < b->bytes; });
-            while (s->bytes_in_use + need > s->cfg.hbm_budget_bytes && !s->pool.empty()) {
-                pm_sim* v = s->pool.back();
-                s->pool.pop_back();
-                s->bytes_in_use -= v->bytes;
-                pm_destroy(v);
-            }
-            if (pm_plan_admission(&need, 1, s->cfg.hbm_budget_bytes - s->bytes_in_use, 1) == 0) return;
-            pm_config c = r->cfg;
-            c.n_particles = capacity_class(r->n);
-            c.device = s->cfg.device;
-            if (pm_create(&c, &h) != PM_OK) {
-                {
-                    std::lock_guard<std::mutex> lk(s->mu);
-                    s->queue.pop_front();
-                }
-                fail_request(s, r, std::string("create: ") + pm_last_error(nullptr));
-                continue;
-            }
-            s->bytes_in_use += h->bytes;
-            s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_use);
-        }
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->queue.pop_front();
-        }
-        r->sim = h;
-        pm_config c = r->cfg;
-        c.n_particles = h->cap;
-        c.device = s->cfg.device;
-        pm_status st = pmi::reconfigure(h, &c);
-        if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);
-        if (st == PM_OK) st = pmi::stage_deposit(h);
-        if (st == PM_OK) st = pm_sync(h);
-        if (st != PM_OK) {
-            fail_request(s, r, h->err);
-            continue;
-        }
-        r->batched = s->cfg.mode == PM_SERVE_BATCHED && r->cfg.n_lattice <= s->cfg.batch_max_lattice &&
-                     r->cfg.force_mode == PM_FORCE_SPECTRAL;
+const float* cached_spectra(pm_server* s, const pm_config& c) {
+    auto it = s->spectra.find(SpectraKey{c.n_lattice, c.mode, c.green_origin, c.deconv_eps});
+    return it == s->spectra.end() ? nullptr : it->second;
+}
+
+void remember_spectra(pm_server* s, const pm_sim* h) {
+    const pm_config& c = h->cfg;
+    const SpectraKey key{c.n_lattice, c.mode, c.green_origin, c.deconv_eps};
+    if (s->spectra.count(key)) return;
+    const size_t b = (size_t)4 * (c.n_lattice / 2 + 1) * c.n_lattice;
+    float* t = nullptr;
+    if (hipMalloc(&t, b) != hipSuccess) return;
+    if (hipMemcpyAsync(t, h->T, b, hipMemcpyDeviceToDevice, s->io_stream) != hipSuccess ||
+        hipStreamSynchronize(s->io_stream) != hipSuccess) {
+        (void)hipFree(t);
+        return;
+    }
+    s->spectra[key] = t;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->bytes_in_use += (int64_t)b;
+    s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_use);
+}
+
+// Admits the queue head if a slot and memory allow.  Returns false when it
+// cannot (queue empty, no slot, or memory held by running requests).
+bool admit_one(pm_server* s) {
+    const int64_t max_active = s->cfg.mode == PM_SERVE_PLAIN ? 1 : std::max(1, s->cfg.max_active);
+    Request* r;
+    int64_t in_use;
+    {
         std::lock_guard<std::mutex> lk(s->mu);
-        r->t_admit = now_s();
-        r->state = PM_REQ_ACTIVE;
-        s->active.push_back(r);
+        if (s->queue.empty() || s->slots_used >= max_active) return false;
+        r = s->queue.front();
+        in_use = s->bytes_in_use;
     }
+    pm_sim* h = take_pooled(s, r);
+    bool pooled = h != nullptr;
+    if (!h) {
+        const int64_t need = sim_bytes(r->cfg);
+        // make room by releasing idle pooled simulations, largest first
+        std::sort(s->pool.begin(), s->pool.end(), [](pm_sim* a, pm_sim* b) { return a->bytes < b->bytes; });
+        while (in_use + need > s->cfg.hbm_budget_bytes && !s->pool.empty()) {
+            pm_sim* v = s->pool.back();
+            s->pool.pop_back();
+            in_use -= v->bytes;
+            destroy_sim(s, v);
+        }
+        if (pm_plan_admission(&need, 1, s->cfg.hbm_budget_bytes - in_use, 1) == 0) {
+            bool idle;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                idle = s->slots_used == 0;
+            }
+            if (idle) {                      // only cached spectra left: release them
+                for (auto& kv : s->spectra) {
+                    (void)hipFree(kv.second);
+                    const int64_t b = (int64_t)4 * (std::get<0>(kv.first) / 2 + 1) * std::get<0>(kv.first);
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->bytes_in_use -= b;
+                    in_use = s->bytes_in_use;
+                }
+                s->spectra.clear();
+            }
+            if (pm_plan_admission(&need, 1, s->cfg.hbm_budget_bytes - in_use, 1) == 0) {
+                s->budget_blocked = true;    // wait for a running request to finish
+                return false;
+            }
+        }
+        pm_config c = r->cfg;
+        c.n_particles = capacity_class(r->n);
+        c.device = s->cfg.device;
+        if (pmi::create(&c, &h, cached_spectra(s, c)) != PM_OK) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->queue.pop_front();
+            }
+            finish_failed(s, r, std::string("create: ") + pm_last_error(nullptr));
+            return true;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->bytes_in_use += h->bytes;
+        s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_use);
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->queue.pop_front();
+        s->slots_used += 1;
+        if (pooled) s->pool_hits += 1;
+    }
+    r->sim = h;
+    h->stream = s->io_stream;
+    pm_config c = r->cfg;
+    c.n_particles = h->cap;
+    c.device = s->cfg.device;
+    pm_status st = pmi::reconfigure(h, &c, cached_spectra(s, c));
+    if (st == PM_OK) remember_spectra(s, h);
+    if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);
+    if (st == PM_OK) st = pmi::stage_deposit(h);
+    if (st == PM_OK) st = pm_sync(h);
+    if (st != PM_OK) {
+        finish_failed(s, r, h->err);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->slots_used -= 1;
+        return true;
+    }
+    r->batched = s->cfg.mode == PM_SERVE_BATCHED && r->cfg.n_lattice <= s->cfg.batch_max_lattice &&
+                 r->n <= s->cfg.batch_max_particles && r->cfg.force_mode == PM_FORCE_SPECTRAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->t_admit = now_s();
+    r->state = PM_REQ_ACTIVE;
+    s->ready.push_back(r);
+    s->cv_work.notify_all();
+    return true;
 }
 
-// ---- batched launches -------------------------------------------------------
+// Reads a finished request back (on the io stream) and returns its
+// simulation to the pool.
+void read_back(pm_server* s, Request* r) {
+    pm_sim* h = r->sim;
+    destroy_graphs(r);
+    if (r->failed) {
+        finish_failed(s, r, r->msg.empty() ? h->err : r->msg);
+    } else {
+        h->stream = s->io_stream;
+        r->x.resize((size_t)r->n);
+        r->y.resize((size_t)r->n);
+        r->vx.resize((size_t)r->n);
+        r->vy.resize((size_t)r->n);
+        if (pm_read(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data()) != PM_OK) {
+            finish_failed(s, r, h->err);
+        } else {
+            r->sim = nullptr;
+            s->pool.push_back(h);
+            r->m.clear();
+            r->m.shrink_to_fit();
+            std::lock_guard<std::mutex> lk(s->mu);
+            r->state = PM_REQ_DONE;
+            r->t_done = now_s();
+            s->completed += 1;
+            s->t_last_done = r->t_done;
+            s->req_lat_ms.push_back((float)((r->t_done - r->t_submit) * 1e3));
+            s->cv_done.notify_all();
+        }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->slots_used -= 1;
+    s->cv_work.notify_all();
+}
 
+void io_main(pm_server* s) {
+    (void)hipSetDevice(s->cfg.device);
+    const int64_t max_active = s->cfg.mode == PM_SERVE_PLAIN ? 1 : std::max(1, s->cfg.max_active);
+    for (;;) {
+        std::vector<Request*> done;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_io.wait(lk, [&] {
+                return !s->finished.empty() ||
+                       (!s->queue.empty() && s->slots_used < max_active && !s->budget_blocked) ||
+                       (s->stop && s->queue.empty() && s->slots_used == 0);
+            });
+            if (s->finished.empty() && s->stop && s->queue.empty() && s->slots_used == 0) break;
+            done.swap(s->finished);
+        }
+        if (!done.empty()) {
+            const double t0 = now_s();
+            for (Request* r : done) read_back(s, r);
+            s->budget_blocked = false;
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->t_readback += now_s() - t0;
+        }
+        // admit until blocked, returning to read-backs as soon as any are waiting
+        const double t0 = now_s();
+        while (!s->budget_blocked && admit_one(s)) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (!s->finished.empty()) break;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->t_admit += now_s() - t0;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cv_work.notify_all();
+}
+
+// ---- worker thread: rounds ------------------------------------------------------
 pmk::FieldDesc field_desc(const pm_sim* h, int rho_index) {
     pmk::FieldDesc d;
     d.rho = h->rho_buf[rho_index];
@@ -400,23 +557,24 @@ pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, 
             const hipError_t e2 = hipGraphInstantiate(&s->plan.graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e2 != hipSuccess) { s->plan.graph = nullptr; return PM_EDEVICE; }
-            s->graph_captures += 1;
+            { std::lock_guard<std::mutex> lk_(s->mu); s->graph_captures += 1; }
         }
         if (hipGraphLaunch(s->plan.graph, s->batch_stream) != hipSuccess) return PM_EDEVICE;
-        s->graph_launches += 1;
+        { std::lock_guard<std::mutex> lk_(s->mu); s->graph_launches += 1; }
     } else {
         if ((st = launch_batched_steps(s, steps)) != PM_OK) return st;
     }
     for (Request* r : members) advance_batched_state(r->sim, steps);
-    s->batched_rounds += 1;
+    { std::lock_guard<std::mutex> lk_(s->mu); s->batched_rounds += 1; }
     return PM_OK;
 }
 
-// A large simulation's round of `steps` steps: replayed from one of two
-// captured graphs (its re-sort swaps the particle buffers, so consecutive
-// rounds alternate) when graphs are on and the round lines up with the
-// re-sort cadence; eager otherwise.  The first round runs eagerly so that
-// launch attributes set lazily by the launchers are in place before capture.
+// A large simulation's round of `steps` steps on h->stream (a lane): replayed
+// from one of two captured graphs (its re-sort swaps the particle buffers, so
+// consecutive rounds alternate) when graphs are on and the round lines up
+// with the re-sort cadence; eager otherwise.  The first round runs eagerly so
+// that launch attributes set lazily by the launchers are in place before
+// capture.  A graph may be replayed on any lane.
 pm_status run_large_round(pm_server* s, Request* r, int steps) {
     pm_sim* h = r->sim;
     r->rounds_done += 1;
@@ -440,10 +598,9 @@ pm_status run_large_round(pm_server* s, Request* r, int steps) {
             if (ok) {
                 r->sorts_per_round = (r->gstate[1].sorts - r->gstate[0].sorts);
                 r->gphase = 0;
-                s->graph_captures += 2;
+                { std::lock_guard<std::mutex> lk_(s->mu); s->graph_captures += 2; }
             } else {
                 destroy_graphs(r);
-                r->graphs_tried = true;
             }
         }
     }
@@ -454,99 +611,97 @@ pm_status run_large_round(pm_server* s, Request* r, int steps) {
         pmi::restore_state(h, r->gstate[k ^ 1]);
         h->sorts = sorts + r->sorts_per_round;
         r->gphase ^= 1;
-        s->graph_launches += 1;
+        { std::lock_guard<std::mutex> lk_(s->mu); s->graph_launches += 1; }
         return PM_OK;
     }
     return pm_step(h, steps);
 }
 
-void complete(pm_server* s, Request* r) {
-    pm_sim* h = r->sim;
-    r->x.resize((size_t)r->n);
-    r->y.resize((size_t)r->n);
-    r->vx.resize((size_t)r->n);
-    r->vy.resize((size_t)r->n);
-    const pm_status st = pm_read(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data());
-    destroy_graphs(r);
-    if (st != PM_OK) {
-        fail_request(s, r, h->err);
-        return;
-    }
-    r->sim = nullptr;
-    s->pool.push_back(h);
-    r->m.clear();
-    r->m.shrink_to_fit();
-    std::lock_guard<std::mutex> lk(s->mu);
-    r->state = PM_REQ_DONE;
-    r->t_done = now_s();
-    s->completed += 1;
-    s->t_last_done = r->t_done;
-    s->req_lat_ms.push_back((float)((r->t_done - r->t_submit) * 1e3));
-    s->cv_done.notify_all();
+
+// Simulated bytes of k steps, for spreading simulations over the lanes.
+double round_cost(const Request* r, int k) {
+    const double n = r->cfg.n_lattice;
+    return (double)k * (36.0 * (double)r->n + 100.0 * n * n + 2e6);
 }
 
+// One round: every admitted request advances min(remaining, round_steps)
+// steps (all remaining steps in PLAIN mode); small ones in batched launches,
+// the others over the lanes, longest first onto the least loaded lane.
 void run_round(pm_server* s) {
     const int R = s->cfg.mode == PM_SERVE_PLAIN ? 0 : std::max(2, s->cfg.round_steps & ~1);
-    std::vector<Request*> batched, large, tail;
-    std::vector<int> advance(s->active.size(), 0);
-    for (size_t i = 0; i < s->active.size(); ++i) {
-        Request* r = s->active[i];
+    std::vector<Request*> batched, others;
+    std::map<Request*, int> advance;
+    for (Request* r : s->active) {
         const int rem = r->steps - r->done_steps;
-        if (R == 0) { advance[i] = rem; tail.push_back(r); continue; }
-        advance[i] = std::min(rem, R);
-        if (rem < R) tail.push_back(r);
-        else if (r->batched) batched.push_back(r);
-        else large.push_back(r);
+        const int k = R == 0 ? rem : std::min(rem, R);
+        advance[r] = k;
+        if (R > 0 && rem >= R && r->batched) batched.push_back(r);
+        else others.push_back(r);
     }
+    std::sort(others.begin(), others.end(),
+              [&](Request* a, Request* b) { return round_cost(a, advance[a]) > round_cost(b, advance[b]); });
+    double load[kLanes] = {};
     const double t0 = now_s();
-    std::vector<std::pair<Request*, pm_status>> errs;
+    std::vector<Request*> bad;
     if (!batched.empty()) {
         s->batch_err = "batched launch failed";
-        const pm_status st = run_batched_round(s, batched, R);
-        if (st != PM_OK)
-            for (Request* r : batched) errs.push_back({r, st});
+        if (run_batched_round(s, batched, R) != PM_OK) {
+            for (Request* r : batched) { r->msg = s->batch_err; bad.push_back(r); }
+            s->plan.signature.clear();
+        }
     }
-    for (Request* r : large) {
-        const pm_status st = run_large_round(s, r, R);
-        if (st != PM_OK) errs.push_back({r, st});
+    for (Request* r : others) {
+        const int l = (int)(std::min_element(load, load + kLanes) - load);
+        load[l] += round_cost(r, advance[r]);
+        r->sim->stream = s->lanes[l];
+        const int rem = r->steps - r->done_steps;
+        const pm_status st = (R > 0 && rem >= R) ? run_large_round(s, r, R) : pm_step(r->sim, advance[r]);
+        if (st != PM_OK) { r->msg = r->sim->err; bad.push_back(r); }
     }
-    for (Request* r : tail) {
-        const pm_status st = pm_step(r->sim, r->steps - r->done_steps);
-        if (st != PM_OK) errs.push_back({r, st});
+    const double t_l = now_s();
+    if (!batched.empty() && hipStreamSynchronize(s->batch_stream) != hipSuccess) {
+        for (Request* r : batched) { r->msg = "device error in batched round"; bad.push_back(r); }
+        s->plan.signature.clear();
     }
-    if (!batched.empty() && hipStreamSynchronize(s->batch_stream) != hipSuccess)
-        for (Request* r : batched) errs.push_back({r, PM_EDEVICE});
-    for (Request* r : s->active) {
-        if (hipStreamSynchronize(r->sim->stream) != hipSuccess) errs.push_back({r, PM_EDEVICE});
-    }
+    for (hipStream_t q : s->lanes)
+        if (hipStreamSynchronize(q) != hipSuccess)
+            for (Request* r : others) { r->msg = "device error"; bad.push_back(r); }
     const double t1 = now_s();
-    s->rounds += 1;
-    std::vector<Request*> keep;
-    for (size_t i = 0; i < s->active.size(); ++i) {
-        Request* r = s->active[i];
-        const bool bad = std::any_of(errs.begin(), errs.end(), [&](const auto& e) { return e.first == r; });
-        if (bad) {
+    std::vector<Request*> keep, out;
+    for (Request* r : s->active) {
+        if (std::find(bad.begin(), bad.end(), r) != bad.end()) {
+            r->failed = true;
             if (r->batched) s->plan.signature.clear();
-            fail_request(s, r, r->batched ? s->batch_err : r->sim->err);
+            out.push_back(r);
             continue;
         }
-        const int k = advance[i];
+        const int k = advance[r];
         r->done_steps += k;
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            const float lat = k > 0 ? (float)((t1 - t0) * 1e3 / k) : 0.f;
-            for (int q = 0; q < k; ++q) s->step_lat_ms.push_back(lat);
-            s->particle_steps += (double)r->n * k;
-            s->steps_done += k;
-        }
         if (r->done_steps >= r->steps) {
             if (r->batched) s->plan.signature.clear();
-            complete(s, r);
+            out.push_back(r);
         } else {
             keep.push_back(r);
         }
     }
     s->active.swap(keep);
+    std::lock_guard<std::mutex> lk(s->mu);
+    const float lat_round = (float)((t1 - t0) * 1e3);
+    for (const auto& [r, k] : advance) {
+        if (r->failed || k <= 0) continue;
+        const float lat = lat_round / (float)k;
+        for (int q = 0; q < k; ++q) s->step_lat_ms.push_back(lat);
+        s->particle_steps += (double)r->n * k;
+        s->steps_done += k;
+    }
+    s->rounds += 1;
+    s->t_launch += t_l - t0;
+    s->t_wait += t1 - t_l;
+    s->n_active = (int64_t)s->active.size();
+    if (!out.empty()) {
+        s->finished.insert(s->finished.end(), out.begin(), out.end());
+        s->cv_io.notify_all();
+    }
 }
 
 void worker_main(pm_server* s) {
@@ -554,11 +709,16 @@ void worker_main(pm_server* s) {
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->active.empty(); });
-            if (s->stop && s->queue.empty() && s->active.empty()) break;
+            s->cv_work.wait(lk, [&] {
+                return !s->ready.empty() || !s->active.empty() ||
+                       (s->stop && s->queue.empty() && s->slots_used == 0);
+            });
+            if (s->ready.empty() && s->active.empty()) break;   // stopping, nothing left
+            s->active.insert(s->active.end(), s->ready.begin(), s->ready.end());
+            s->ready.clear();
+            s->n_active = (int64_t)s->active.size();
         }
-        admit(s);
-        if (!s->active.empty()) run_round(s);
+        run_round(s);
     }
 }
 
@@ -575,6 +735,7 @@ void pm_server_config_default(pm_server_config* c) {
     c->round_steps = 8;
     c->use_graphs = 1;
     c->batch_max_lattice = 512;
+    c->batch_max_particles = 1 << 17;
     c->device = 0;
 }
 
@@ -587,6 +748,20 @@ int32_t pm_plan_admission(const int64_t* need_bytes, int32_t n_queued, int64_t f
     return k;
 }
 
+static void free_server(pm_server* s) {
+    for (auto* list : {&s->active, &s->ready, &s->finished})
+        for (Request* r : *list)
+            if (r->sim) { destroy_graphs(r); pm_destroy(r->sim); r->sim = nullptr; }
+    for (pm_sim* h : s->pool) pm_destroy(h);
+    for (auto& kv : s->spectra) (void)hipFree(kv.second);
+    if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
+    if (s->dscratch) (void)hipFree(s->dscratch);
+    if (s->hscratch) (void)hipHostFree(s->hscratch);
+    for (hipStream_t* q : {&s->batch_stream, &s->io_stream, &s->lanes[0], &s->lanes[1]})
+        if (*q) (void)hipStreamDestroy(*q);
+    delete s;
+}
+
 pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     if (!cfg || !out) return PM_EINVAL;
     *out = nullptr;
@@ -597,12 +772,15 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     pm_server* s = new (std::nothrow) pm_server();
     if (!s) return PM_ENOMEM;
     s->cfg = *cfg;
-    if (hipSetDevice(cfg->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->batch_stream, hipStreamNonBlocking) != hipSuccess) {
-        delete s;
+    bool ok = hipSetDevice(cfg->device) == hipSuccess;
+    for (hipStream_t* q : {&s->batch_stream, &s->lanes[0], &s->lanes[1], &s->io_stream})
+        ok = ok && hipStreamCreateWithFlags(q, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        free_server(s);
         return PM_EDEVICE;
     }
     s->worker = std::thread(worker_main, s);
+    s->io = std::thread(io_main, s);
     *out = s;
     return PM_OK;
 }
@@ -635,6 +813,10 @@ pm_status pm_server_submit(pm_server_handle s, const pm_config* cfg, const float
     r->vy.assign(vy, vy + n);
     r->m.assign(m, m + n);
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+        s->err = "server is stopping";
+        return PM_ECANCELLED;
+    }
     r->id = s->next_id++;
     r->t_submit = now_s();
     if (s->t_first_submit < 0) s->t_first_submit = r->t_submit;
@@ -642,7 +824,7 @@ pm_status pm_server_submit(pm_server_handle s, const pm_config* cfg, const float
     s->queue.push_back(r.get());
     s->reqs[r->id] = std::move(r);
     s->submitted += 1;
-    s->cv_work.notify_all();
+    s->cv_io.notify_all();
     return PM_OK;
 }
 
@@ -659,11 +841,17 @@ pm_status pm_server_wait(pm_server_handle s, int64_t id, float* x, float* y, flo
     if (!s) return PM_EINVAL;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return PM_EINVAL;
+    if (it == s->reqs.end()) {
+        s->err = "unknown request id";
+        return PM_EINVAL;
+    }
     Request* r = it->second.get();
     auto done = [&] { return r->state == PM_REQ_DONE || r->state == PM_REQ_FAILED; };
     if (timeout_s > 0) {
-        if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return PM_EBUSY;
+        if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
+            s->err = "timed out";
+            return PM_EBUSY;
+        }
     } else {
         s->cv_done.wait(lk, done);
     }
@@ -695,7 +883,7 @@ pm_status pm_server_get_stats(pm_server_handle s, pm_server_stats* o) {
     o->submitted = s->submitted;
     o->completed = s->completed;
     o->failed = s->failed;
-    o->active = (int64_t)s->active.size();
+    o->active = s->n_active;
     o->queued = (int64_t)s->queue.size();
     o->particle_steps = s->particle_steps;
     o->steps = s->steps_done;
@@ -712,6 +900,10 @@ pm_status pm_server_get_stats(pm_server_handle s, pm_server_stats* o) {
     o->graph_launches = s->graph_launches;
     o->graph_captures = s->graph_captures;
     o->pool_hits = s->pool_hits;
+    o->admit_s = s->t_admit;
+    o->launch_s = s->t_launch;
+    o->readback_s = s->t_readback;
+    o->wait_s = s->t_wait;
     return PM_OK;
 }
 
@@ -727,18 +919,13 @@ pm_status pm_server_destroy(pm_server_handle s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        s->cv_io.notify_all();
         s->cv_work.notify_all();
     }
+    if (s->io.joinable()) s->io.join();
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->cfg.device);
-    for (Request* r : s->active)
-        if (r->sim) { destroy_graphs(r); pm_destroy(r->sim); }
-    for (pm_sim* h : s->pool) pm_destroy(h);
-    if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
-    if (s->dscratch) (void)hipFree(s->dscratch);
-    if (s->hscratch) (void)hipHostFree(s->hscratch);
-    if (s->batch_stream) (void)hipStreamDestroy(s->batch_stream);
-    delete s;
+    free_server(s);
     return PM_OK;
 }
 

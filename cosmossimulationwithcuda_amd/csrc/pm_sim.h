@@ -21,7 +21,8 @@ struct ProfRec {
 
 struct pm_sim {
     pm_config cfg{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;          // stream the handle's work is issued on
+    hipStream_t own_stream = nullptr;      // created with the handle (a server may issue on its own streams)
     int64_t cap = 0, np = 0;
     int shift = 40;
     pmk::ParticleArrays cur{}, alt{};      // sorted state and the counting-sort target / read scratch
@@ -75,10 +76,14 @@ pm_status stage_deposit(pm_sim* h);
 pm_status stage_field(pm_sim* h);
 pm_status stage_push(pm_sim* h);
 pm_status stage_push_deposit(pm_sim* h);
+// pm_create with an optional ready transfer function (device, [N/2+1][N]
+// floats, built for the same lattice, mode, G(0) and eps) copied instead of
+// rebuilt.
+pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src);
 // Re-targets an existing handle (same lattice size, capacity >= its needs) to
-// another configuration: rebuilds the transfer function and the stencil symbol
-// when the physics parameters differ.
-pm_status reconfigure(pm_sim* h, const pm_config* cfg);
+// another configuration: rebuilds (or copies from T_src) the transfer function
+// and the stencil symbol when the physics parameters differ.
+pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src);
 
 // Host-side state that pm_step advances (for replaying captured step graphs).
 struct HostState {

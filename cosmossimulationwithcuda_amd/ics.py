@@ -106,3 +106,47 @@ def plummer_binary(n, seed, n_lattice=4096, a=64.0, separation=800.0, gravity=1.
         vy = np.full(nk, sgn * vorb)
         out.append(_f32(x, y, vx, vy, np.ones(nk)))
     return tuple(np.concatenate([p[i] for p in out]) for i in range(5))
+
+
+# ----- serving ensemble -------------------------------------------------------------
+# Requests of the serving workload are disks centred on their lattice, in
+# lattice-scaled units: positions are s * (template), s = n_lattice / 16, and
+# with unit masses and G = 0.25 * s / n the circular speed does not depend on
+# the particle count n or the lattice (v = 0.5 * template speed, <= ~0.3
+# cells per time unit).  Templates hold i.i.d. draws, so a contiguous slice at
+# a seeded offset is itself a seeded sample of the disk.
+
+def disk_template(kind, n, seed, dispersion=0.02):
+    """Unit-scale disk of n particles about the origin, circular speeds for
+    total mass 1 and G = 1 (1/r force law).  kind: "exponential" (r_d = 1,
+    truncated at 5) or "uniform" (radius 4)."""
+    rng = np.random.default_rng(seed)
+    if kind == "exponential":
+        r = _disk_radii(rng, n, 1.0, 5.0)
+        frac = _enclosed_fraction(r, 1.0, 5.0)
+    elif kind == "uniform":
+        r = 4.0 * np.sqrt(rng.uniform(0.0, 1.0, n))
+        frac = (r / 4.0) ** 2
+    else:
+        raise ValueError(kind)
+    a = rng.uniform(0.0, 2.0 * np.pi, n)
+    vc = np.sqrt(frac / np.maximum(r, 1e-3))
+    ca, sa = np.cos(a), np.sin(a)
+    vx = -vc * sa + dispersion * vc * rng.standard_normal(n)
+    vy = vc * ca + dispersion * vc * rng.standard_normal(n)
+    return _f32(r * ca, r * sa, vx, vy)
+
+
+def ensemble_request(template, n_lattice, n, seed):
+    """(x, y, vx, vy, m, gravity) of one serving request: n particles sliced
+    from a disk template at a seeded offset, centred on an n_lattice lattice
+    (with a seeded sub-cell jitter of the centre)."""
+    tx, ty, tvx, tvy = template
+    rng = np.random.default_rng(seed)
+    off = int(rng.integers(0, tx.size - n + 1))
+    s = np.float32(n_lattice / 16.0)
+    cx, cy = np.float32(n_lattice / 2.0 + rng.uniform(-0.5, 0.5)), np.float32(n_lattice / 2.0 + rng.uniform(-0.5, 0.5))
+    sl = slice(off, off + n)
+    half = np.float32(0.5)
+    return (cx + s * tx[sl], cy + s * ty[sl], half * tvx[sl], half * tvy[sl], np.ones(n, np.float32),
+            0.25 * float(s) / n)

@@ -276,7 +276,8 @@ class ServerConfig(ctypes.Structure):
         ("use_graphs", ctypes.c_int32),
         ("batch_max_lattice", ctypes.c_int32),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("batch_max_particles", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -286,7 +287,8 @@ class ServerStats(ctypes.Structure):
                                        "step_latency_p50_ms", "step_latency_p99_ms",
                                        "request_latency_p50_ms", "request_latency_p99_ms")] + [
         (k, ctypes.c_int64) for k in ("bytes_in_use", "peak_bytes", "rounds", "batched_launch_rounds",
-                                      "graph_launches", "graph_captures", "pool_hits")]
+                                      "graph_launches", "graph_captures", "pool_hits")] + [
+        (k, ctypes.c_double) for k in ("admit_s", "launch_s", "readback_s", "wait_s")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -354,13 +356,14 @@ class Server:
         self._check(lib().pm_server_status(self._h, rid, ctypes.byref(st)))
         return st.value
 
-    def wait(self, rid, timeout_s=0.0):
-        """Final (x, y, vx, vy) of a request, in load order."""
+    def wait(self, rid, timeout_s=0.0, copy=True):
+        """Final (x, y, vx, vy) of a request, in load order (None with copy=False)."""
         n = self._n[rid]
-        out = [np.empty(n, np.float32) for _ in range(4)]
-        self._check(lib().pm_server_wait(self._h, rid, *[o.ctypes.data for o in out], float(timeout_s)))
+        out = [np.empty(n, np.float32) for _ in range(4)] if copy else None
+        ptrs = [o.ctypes.data for o in out] if copy else [None] * 4
+        self._check(lib().pm_server_wait(self._h, rid, *ptrs, float(timeout_s)))
         del self._n[rid]
-        return tuple(out)
+        return tuple(out) if copy else None
 
     def drain(self):
         self._check(lib().pm_server_drain(self._h))

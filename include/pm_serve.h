@@ -13,8 +13,9 @@
  * Modes:
  *   PM_SERVE_PLAIN    one request at a time, each stepped on its own to the
  *                     end (the one-at-a-time loop baseline);
- *   PM_SERVE_BATCHED  many requests resident; simulations with lattices up to
- *                     `batch_max_lattice` are advanced by batched launches
+ *   PM_SERVE_BATCHED  many requests resident; small simulations (lattice up to
+ *                     `batch_max_lattice`, particles up to
+ *                     `batch_max_particles`) are advanced by batched launches
  *                     (one launch per pass for all of them), larger ones on
  *                     their own streams, concurrently, each round replayed
  *                     from captured HIP graphs when `use_graphs` is set.
@@ -42,9 +43,10 @@ typedef struct pm_server_config {
     int32_t mode;               /* PM_SERVE_* */
     int32_t round_steps;        /* steps per round (even; default 8) */
     int32_t use_graphs;         /* replay rounds from captured HIP graphs */
-    int32_t batch_max_lattice;  /* lattices up to this size run in batched launches (BATCHED) */
+    int32_t batch_max_lattice;  /* lattices up to this size ... */
     int32_t device;
-    int32_t reserved[8];
+    int32_t batch_max_particles; /* ... with at most this many particles run in batched launches (BATCHED) */
+    int32_t reserved[7];
 } pm_server_config;
 
 typedef struct pm_server_stats {
@@ -57,6 +59,9 @@ typedef struct pm_server_stats {
     double request_latency_p50_ms, request_latency_p99_ms; /* submit -> result */
     int64_t bytes_in_use, peak_bytes;
     int64_t rounds, batched_launch_rounds, graph_launches, graph_captures, pool_hits;
+    /* worker-thread wall time by phase (s): admission (allocation, upload, first
+     * deposit), launching rounds, reading back results, waiting for the device */
+    double admit_s, launch_s, readback_s, wait_s;
 } pm_server_stats;
 
 void pm_server_config_default(pm_server_config* cfg);

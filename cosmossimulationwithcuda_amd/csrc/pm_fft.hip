@@ -28,7 +28,8 @@
 // FFT: in-place Stockham on LDS, every thread of the block working across all
 // transforms of the block; stages of radix 4 or 8 (plan choice) after one
 // leading radix-2/4 stage for the leftover factor; twiddles exp(-2 pi i m/N)
-// from a table built in FP64.
+// from a quarter-wave table built in FP64 and staged in LDS; LDS rows are
+// padded against bank conflicts.
 //
 // Every pass reads its arrays from a FieldDesc; blockIdx.y selects the
 // instance, so one launch serves one simulation or a batch of simulations of
@@ -102,10 +103,51 @@ __device__ __forceinline__ void dft(float2* v) {
     else dft8<INV>(v);
 }
 
+// LDS layout: element i of a block's buffers is stored at pad(i) = i + i/32
+// (one spare slot per 32 elements), which spreads the strided Stockham writes
+// of the early stages over the LDS banks; strides that are multiples of 32
+// elements stay compile-time offsets.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
+constexpr int padded(int n) { return n + n / 32; }
+
+// Twiddles exp(-2 pi i m / N) for m in [0, N) from the quarter table
+// q[r] = exp(-2 pi i r / N), r < N/4, held in LDS: the quadrant factor
+// (-i)^(m / (N/4)) is a swap and sign change, so the values are exactly the
+// table's.
+template <int N, bool INV>
+__device__ __forceinline__ float2 twiddle(const float2* q, int m) {
+    constexpr int Q4 = N / 4, LQ = ilog2c(Q4);
+    const float2 w = q[m & (Q4 - 1)];
+    const unsigned qd = (unsigned)(m >> LQ) & 3u;
+    // quadrants 1 and 3 swap the components; signs: x flips for 2, 3; y for 1, 2
+    const bool sw = qd & 1u;
+    const unsigned sx = (qd >> 1) << 31, sy = ((qd ^ (qd >> 1)) & 1u) << 31;
+    const float x = sw ? w.y : w.x, y = sw ? w.x : w.y;
+    return make_float2(__uint_as_float(__float_as_uint(x) ^ sx),
+                       __uint_as_float(__float_as_uint(y) ^ sy ^ (INV ? 0x80000000u : 0u)));
+}
+
+// Row-group index of a row-pass block.  Workgroups are dispatched round-robin
+// over the 8 XCDs; the row passes touch the transposed half spectrum in short
+// segments (2*RP rows per kx), so consecutive row groups are given to
+// consecutive workgroups of the same XCD, whose L2 then serves the rest of
+// each 128-byte line to the neighbouring groups.
+__device__ __forceinline__ int row_group() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// Loads the quarter twiddle table into LDS (callers synchronise after).
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* q, const float2* __restrict__ tw) {
+    for (int e = threadIdx.x; e < N / 4; e += kBlock) q[e] = tw[e];
+}
+
 // One Stockham stage of radix R with sub-transform length NS over NFFT
-// in-place transforms of length N at buf + f*N.  Reads, barrier, writes, barrier.
+// in-place transforms of length N at buf + f*N (padded).  Reads, barrier,
+// writes, barrier.
 template <int N, int NFFT, int R, int NS, bool INV>
-__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw) {
+__device__ __forceinline__ void stage(float2* buf, const float2* tw) {
     constexpr int Q = N / R, NB = NFFT * Q, PER = (NB + kBlock - 1) / kBlock;
     constexpr int STEP = N / (R * NS);
     const int t = threadIdx.x;
@@ -116,18 +158,19 @@ __device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw
         const int bb = t + p * kBlock;
         if (NB % kBlock == 0 || bb < NB) {
             const int f = bb / Q, b = bb - f * Q;
-            const float2* base = buf + f * N;
+            if constexpr (Q % 32 == 0) {
+                const float2* src = buf + pad(f * N + b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = base[b + r * Q];
+                for (int r = 0; r < R; ++r) v[p][r] = src[r * padded(Q)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[p][r] = buf[pad(f * N + b + r * Q)];
+            }
             const int k = b & (NS - 1);
             if (NS > 1) {
                 const int e = k * STEP;
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    float2 w = tw[r * e];
-                    if (INV) w.y = -w.y;
-                    v[p][r] = cmul(v[p][r], w);
-                }
+                for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], twiddle<N, INV>(tw, r * e));
             }
             dft<R, INV>(v[p]);
             ob[p] = f * N + (b - k) * R + k;
@@ -138,27 +181,36 @@ __device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw
     for (int p = 0; p < PER; ++p) {
         const int bb = t + p * kBlock;
         if (NB % kBlock == 0 || bb < NB) {
-            float2* o = buf + ob[p];
+            if constexpr (NS % 32 == 0) {
+                float2* dst = buf + pad(ob[p]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) o[r * NS] = v[p][r];
+                for (int r = 0; r < R; ++r) dst[r * padded(NS)] = v[p][r];
+            } else if constexpr (32 % (NS * R) == 0) {   // the R outputs share one 32-element row
+                float2* dst = buf + pad(ob[p]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * NS] = v[p][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[pad(ob[p] + r * NS)] = v[p][r];
+            }
         }
     }
     __syncthreads();
 }
 
 template <int N, int NFFT, int R, int NS, bool INV>
-__device__ __forceinline__ void stages_from(float2* buf, const float2* __restrict__ tw) {
+__device__ __forceinline__ void stages_from(float2* buf, const float2* tw) {
     if constexpr (NS < N) {
         stage<N, NFFT, R, NS, INV>(buf, tw);
         stages_from<N, NFFT, R, NS * R, INV>(buf, tw);
     }
 }
 
-// NFFT in-place transforms of length N at buf + f*N, radix RADIX (4 or 8)
-// after a leading stage for the leftover factor.  Caller synchronises before;
-// ends with a barrier.
+// NFFT in-place transforms of length N at buf + f*N (padded), radix RADIX
+// (4 or 8) after a leading stage for the leftover factor; tw is the LDS quarter
+// table.  Caller synchronises before; ends with a barrier.
 template <int N, int NFFT, int RADIX, bool INV>
-__device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ tw) {
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
     constexpr int LR = ilog2c(RADIX), L = ilog2c(N), REM = L % LR;
     if constexpr (REM == 0) {
         stages_from<N, NFFT, RADIX, 1, INV>(buf, tw);
@@ -173,22 +225,24 @@ __device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ 
 template <int N, int RP, int RADIX>
 __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict__ descs,
                                                      const float2* __restrict__ tw) {
-    __shared__ float2 buf[RP * N];
+    __shared__ float2 buf[padded(RP * N)];
+    __shared__ float2 twl[N / 4];
     const FieldDesc d = descs[blockIdx.y];
     const int t = threadIdx.x;
-    const int j0 = blockIdx.x * 2 * RP;
+    const int j0 = row_group() * 2 * RP;
+    load_twiddles<N>(twl, tw);
     for (int e = t; e < RP * N; e += kBlock) {
         const int f = e / N, i = e - f * N;
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
-        buf[e] = make_float2((float)d.rho[ra] * d.inv_scale, (float)d.rho[ra + N] * d.inv_scale);
+        buf[pad(e)] = make_float2((float)d.rho[ra] * d.inv_scale, (float)d.rho[ra + N] * d.inv_scale);
     }
     __syncthreads();
-    lds_fft<N, RP, RADIX, false>(buf, tw);
+    lds_fft<N, RP, RADIX, false>(buf, twl);
     constexpr int NK = N / 2 + 1;
     for (int e = t; e < NK * RP; e += kBlock) {
         const int k = e / RP, f = e - k * RP;
-        const float2 z = buf[f * N + k];
-        const float2 zc = buf[f * N + ((N - k) & (N - 1))];
+        const float2 z = buf[pad(f * N + k)];
+        const float2 zc = buf[pad(f * N + ((N - k) & (N - 1)))];
         // Xa = (Z + conj(Zc))/2,  Xb = (Z - conj(Zc))/(2i)
         const float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
         const float2 xb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
@@ -198,37 +252,75 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
 }
 
 // --------------------------------------------------------------- column pass --
+// Blocks cover kx = blockIdx.x * CB + f, f < CB, for kx in [0, N/2).  The
+// columns kx = 0 and kx = N/2 are real (both rows of a row-pair FFT are real
+// there), so slot kx = 0 carries both as one complex column z = H0 + i*HN2:
+// its spectrum Z splits into A = (Z + conj Z(-k))/2 and B = (Z - conj Z(-k))/2i,
+// and W = A*T0 + i*B*TN2 transforms back to the two real results (real and
+// imaginary parts).  N/2 blocks of CB = 1 then fill the device evenly.
 template <int N, int CB, int RADIX, bool FORCE>
 __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict__ descs,
                                                       const float2* __restrict__ tw) {
-    __shared__ float2 buf[(FORCE ? 2 : 1) * CB * N];
+    __shared__ float2 buf[padded((FORCE ? 2 : 1) * CB * N)];
+    __shared__ float2 twl[N / 4];
     const FieldDesc d = descs[blockIdx.y];
-    constexpr int NK = N / 2 + 1;
+    constexpr int NY = N / 2;                 // the Nyquist column's kx
     const int t = threadIdx.x;
     const int k0 = blockIdx.x * CB;
+    load_twiddles<N>(twl, tw);
     for (int e = t; e < CB * N; e += kBlock) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
-        buf[e] = k < NK ? d.H[(size_t)k * N + ky] : make_float2(0.f, 0.f);
+        buf[pad(e)] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : d.H[(size_t)k * N + ky];
     }
     __syncthreads();
-    lds_fft<N, CB, RADIX, false>(buf, tw);
-    for (int e = t; e < CB * N; e += kBlock) {
+    lds_fft<N, CB, RADIX, false>(buf, twl);
+    for (int e = (k0 == 0 ? N : 0) + t; e < CB * N; e += kBlock) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
-        const float tv = k < NK ? d.T[(size_t)k * N + ky] : 0.f;
-        const float2 y = make_float2(buf[e].x * tv, buf[e].y * tv);
-        buf[e] = y;
+        const float tv = d.T[(size_t)k * N + ky];
+        const float2 x = buf[pad(e)];
+        const float2 y = make_float2(x.x * tv, x.y * tv);
+        buf[pad(e)] = y;
         if (FORCE) {
             const float dv = d.dk[ky];
-            buf[CB * N + e] = make_float2(-y.y * dv, y.x * dv);   // i * d * y
+            buf[pad(CB * N + e)] = make_float2(-y.y * dv, y.x * dv);   // i * d * y
+        }
+    }
+    if (k0 == 0) {                            // packed kx = 0 / N/2 slot, by conjugate pairs
+        for (int a = t; a <= N / 2; a += kBlock) {
+            const int b = (N - a) & (N - 1);
+            const float2 za = buf[pad(a)], zb = buf[pad(b)];
+            const float2 A = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+            const float2 B = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
+            const float t0a = d.T[a], t0b = d.T[b];
+            const float tna = d.T[(size_t)NY * N + a], tnb = d.T[(size_t)NY * N + b];
+            // W(a) = A T0 + i B TN;  W(b) = conj(A) T0 + i conj(B) TN
+            const float2 wa = make_float2(A.x * t0a - B.y * tna, A.y * t0a + B.x * tna);
+            const float2 wb = make_float2(A.x * t0b + B.y * tnb, -A.y * t0b + B.x * tnb);
+            buf[pad(a)] = wa;
+            buf[pad(b)] = wb;
+            if (FORCE) {
+                const float da = d.dk[a], db = d.dk[b];
+                buf[pad(CB * N + a)] = make_float2(-wa.y * da, wa.x * da);
+                buf[pad(CB * N + b)] = make_float2(-wb.y * db, wb.x * db);
+            }
         }
     }
     __syncthreads();
-    lds_fft<N, (FORCE ? 2 : 1) * CB, RADIX, true>(buf, tw);
+    lds_fft<N, (FORCE ? 2 : 1) * CB, RADIX, true>(buf, twl);
     for (int e = t; e < CB * N; e += kBlock) {
         const int f = e / N, j = e - f * N, k = k0 + f;
-        if (k < NK) {
-            d.H[(size_t)k * N + j] = buf[e];
-            if (FORCE) d.Hfy[(size_t)k * N + j] = buf[CB * N + e];
+        const float2 w = buf[pad(e)];
+        const float2 v = FORCE ? buf[pad(CB * N + e)] : make_float2(0.f, 0.f);
+        if (k == 0) {
+            d.H[j] = make_float2(w.x, 0.f);
+            d.H[(size_t)NY * N + j] = make_float2(w.y, 0.f);
+            if (FORCE) {
+                d.Hfy[j] = make_float2(v.x, 0.f);
+                d.Hfy[(size_t)NY * N + j] = make_float2(v.y, 0.f);
+            }
+        } else {
+            d.H[(size_t)k * N + j] = w;
+            if (FORCE) d.Hfy[(size_t)k * N + j] = v;
         }
     }
 }
@@ -237,26 +329,28 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
 template <int N, int RP, int RADIX, bool FORCE>
 __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict__ descs,
                                                      const float2* __restrict__ tw) {
-    __shared__ float2 buf[(FORCE ? 2 : 1) * RP * N];
+    __shared__ float2 buf[padded((FORCE ? 2 : 1) * RP * N)];
+    __shared__ float2 twl[N / 4];
     const FieldDesc d = descs[blockIdx.y];
     constexpr int NK = N / 2 + 1;
     const int t = threadIdx.x;
-    const int j0 = blockIdx.x * 2 * RP;
+    const int j0 = row_group() * 2 * RP;
+    load_twiddles<N>(twl, tw);
     // potential: rows (j0+2f, j0+2f+1) packed as one complex row
     for (int e = t; e < NK * RP; e += kBlock) {
         const int k = e / RP, f = e - k * RP;
         const float4 v = *reinterpret_cast<const float4*>(d.H + (size_t)k * N + j0 + 2 * f);
         float2 xa = make_float2(v.x, v.y), xb = make_float2(v.z, v.w);
         if (k == 0 || k == N / 2) { xa.y = 0.f; xb.y = 0.f; }
-        buf[f * N + k] = make_float2(xa.x - xb.y, xa.y + xb.x);
-        if (k > 0 && k < N / 2) buf[f * N + N - k] = make_float2(xa.x + xb.y, xb.x - xa.y);
+        buf[pad(f * N + k)] = make_float2(xa.x - xb.y, xa.y + xb.x);
+        if (k > 0 && k < N / 2) buf[pad(f * N + N - k)] = make_float2(xa.x + xb.y, xb.x - xa.y);
     }
     __syncthreads();
-    lds_fft<N, RP, RADIX, true>(buf, tw);
+    lds_fft<N, RP, RADIX, true>(buf, twl);
     for (int e = t; e < RP * N; e += kBlock) {
         const int f = e / N, i = e - f * N;
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
-        const float2 z = buf[e];
+        const float2 z = buf[pad(e)];
         d.phi[ra] = z.x;
         d.phi[ra + N] = z.y;
     }
@@ -272,14 +366,14 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
         const float dv = d.dk[k];
         const float2 fx = make_float2(-p.y * dv, p.x * dv);     // i * d(kx) * Phi-hat
         if (k == 0 || k == N / 2) fy.y = 0.f;
-        buf[q * N + k] = make_float2(fx.x - fy.y, fx.y + fy.x);
-        if (k > 0 && k < N / 2) buf[q * N + N - k] = make_float2(fx.x + fy.y, fy.x - fx.y);
+        buf[pad(q * N + k)] = make_float2(fx.x - fy.y, fx.y + fy.x);
+        if (k > 0 && k < N / 2) buf[pad(q * N + N - k)] = make_float2(fx.x + fy.y, fy.x - fx.y);
     }
     __syncthreads();
-    lds_fft<N, R2, RADIX, true>(buf, tw);
+    lds_fft<N, R2, RADIX, true>(buf, twl);
     for (int e = t; e < R2 * N; e += kBlock) {
         const int q = e / N, i = e - q * N;
-        d.force[(size_t)(j0 + q) * N + i] = buf[e];
+        d.force[(size_t)(j0 + q) * N + i] = buf[pad(e)];
     }
 }
 
@@ -344,7 +438,7 @@ hipError_t r2c_t(const FieldDesc* d, int count, const float2* tw, hipStream_t s)
 
 template <int N, int CB, int RADIX>
 hipError_t col_t(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
-    const dim3 grid((N / 2 + 1 + CB - 1) / CB, count);
+    const dim3 grid(N / 2 / CB, count);
     if (force) hipLaunchKernelGGL((k_col_conv<N, CB, RADIX, true>), grid, dim3(kBlock), 0, s, d, tw);
     else hipLaunchKernelGGL((k_col_conv<N, CB, RADIX, false>), grid, dim3(kBlock), 0, s, d, tw);
     return hipGetLastError();
@@ -359,12 +453,13 @@ hipError_t c2r_t(const FieldDesc* d, int count, const float2* tw, bool force, hi
 }
 
 // Row batches RP in {1,2,4} and column batches CB in {1,2,4,8} are compiled
-// where the LDS footprint (2*RP*N*8 or 2*CB*N*8 bytes with the force pass)
-// fits 160 KB; plan_valid() states the same rule for the host.
+// where the LDS footprint (2*RP*N*8 or 2*CB*N*8 bytes with the force pass,
+// plus the 2*N-byte quarter twiddle table) fits 160 KB; plan_valid() states
+// the same rule for the host (with the 1/32 LDS padding).
 template <int N, int RP>
-constexpr bool rp_ok() { return 2 * RP * N * 8 <= 160 * 1024 && 2 * RP <= N; }
+constexpr bool rp_ok() { return padded(2 * RP * N) * 8 + 2 * N <= 160 * 1024 && 2 * RP <= N; }
 template <int N, int CB>
-constexpr bool cb_ok() { return 2 * CB * N * 8 <= 160 * 1024 && CB <= N / 2 + 1; }
+constexpr bool cb_ok() { return padded(2 * CB * N) * 8 + 2 * N <= 160 * 1024 && CB <= N / 2; }
 
 template <int N>
 hipError_t r2c_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s) {
@@ -425,15 +520,17 @@ bool plan_valid(int n, FftPlan p) {
     if (p.radix != 4 && p.radix != 8) return false;
     if (p.rp != 1 && p.rp != 2 && p.rp != 4) return false;
     if (p.cb != 1 && p.cb != 2 && p.cb != 4 && p.cb != 8) return false;
-    const long lds_r = 2L * p.rp * n * 8, lds_c = 2L * p.cb * n * 8;
-    return lds_r <= 160 * 1024 && lds_c <= 160 * 1024 && 2 * p.rp <= n && p.cb <= n / 2 + 1;
+    const long lds_r = padded(2 * p.rp * n) * 8L + 2L * n, lds_c = padded(2 * p.cb * n) * 8L + 2L * n;
+    return lds_r <= 160 * 1024 && lds_c <= 160 * 1024 && 2 * p.rp <= n && p.cb <= n / 2;
 }
 
+// Measured on MI355X over n = 256..4096: one row pair / one column per block
+// (the most blocks in flight) is fastest at every size; radix 8 from 2048 up.
 FftPlan default_plan(int n) {
     FftPlan p;
-    p.radix = 4;
-    p.rp = n >= 8192 ? 1 : 2;
-    p.cb = n >= 1024 ? 1 : (n >= 512 ? 2 : 4);
+    p.radix = n >= 2048 ? 8 : 4;
+    p.rp = 1;
+    p.cb = 1;
     return p;
 }
 

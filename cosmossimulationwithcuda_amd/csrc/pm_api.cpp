@@ -49,13 +49,28 @@ pm_status validate(const pm_config* c, std::string* why) {
     if (!(c->deconv_eps >= 0.0) || !std::isfinite(c->green_origin)) { *why = "bad deconv_eps or green_origin"; return PM_EINVAL; }
     if (c->wrap_mode != PM_WRAP_KEEP_VELOCITY && c->wrap_mode != PM_WRAP_REFERENCE) { *why = "bad wrap_mode"; return PM_EINVAL; }
     if (c->force_mode != PM_FORCE_SPECTRAL && c->force_mode != PM_FORCE_STENCIL) { *why = "bad force_mode"; return PM_EINVAL; }
-    if (c->precision != PM_FFT_FP32) { *why = "precision: only PM_FFT_FP32 is built"; return PM_EINVAL; }
+    if (c->precision != PM_FFT_FP32 && c->precision != PM_FFT_BF16X3) { *why = "precision: unknown"; return PM_EINVAL; }
+    if (c->precision == PM_FFT_BF16X3 && (c->n_lattice < 256 || c->n_lattice > 8192)) {
+        *why = "precision PM_FFT_BF16X3 needs 256 <= n_lattice <= 8192";
+        return PM_EINVAL;
+    }
     if (c->mass_shift < 0 || c->mass_shift > 62) { *why = "mass_shift must be in [0, 62]"; return PM_EINVAL; }
     if (c->wrap_mode == PM_WRAP_REFERENCE && c->dt == 0.0f) { *why = "wrap_mode reference needs dt != 0"; return PM_EINVAL; }
     return PM_OK;
 }
 
 int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+// FP32: Stockham radix 4/8 (default plan for the size); BF16X3: radix-16
+// matrix-core stages, one row pair / column per block.
+pmk::FftPlan plan_for(const pm_config& c) {
+    if (c.precision == PM_FFT_BF16X3) {
+        pmk::FftPlan p;
+        p.rp = 1; p.cb = 1; p.radix = 16;
+        return p;
+    }
+    return pmk::default_plan(c.n_lattice);
+}
 
 struct Layout {
     int64_t particles, bins, rho, fdesc, H, T, dk, tw, phi, force, diag, total, setup_work;
@@ -325,6 +340,7 @@ pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src) {
     const bool spectra = cfg->mode != h->cfg.mode || cfg->green_origin != h->cfg.green_origin ||
                          cfg->deconv_eps != h->cfg.deconv_eps;
     const bool tables = cfg->gravity != h->cfg.gravity;
+    if (cfg->precision != h->cfg.precision) h->plan = plan_for(*cfg);
     const int64_t cap = h->cap;
     h->cfg = *cfg;
     h->cfg.n_particles = cap;
@@ -445,7 +461,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
     h->rho_buf[1] = h->rho_buf[0] + L.rho / 8;
     if ((st = dalloc(h, &p, L.fdesc)) != PM_OK) return bail(st);
     h->fdesc = static_cast<pmk::FieldDesc*>(p);
-    h->plan = pmk::default_plan(n);
+    h->plan = plan_for(*cfg);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->Hfy = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.T)) != PM_OK) return bail(st); h->T = static_cast<float*>(p);
@@ -657,6 +673,8 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         pmk::FftPlan p = h->plan;
         (k == "fft_rp" ? p.rp : k == "fft_cb" ? p.cb : p.radix) = (int)value;
         if (!pmk::plan_valid(h->cfg.n_lattice, p)) return fail(h, PM_EINVAL, "FFT plan not available for this lattice");
+        if ((p.radix == 16) != (h->cfg.precision == PM_FFT_BF16X3))
+            return fail(h, PM_EINVAL, "fft_radix 16 is the PM_FFT_BF16X3 precision; 4 or 8 are FP32");
         h->plan = p;
         return PM_OK;
     }
@@ -708,7 +726,7 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
     h->profiling = false;
     pm_status st = PM_OK;
     const int n = h->cfg.n_lattice;
-    if (flags & PM_TUNE_FFT) {
+    if ((flags & PM_TUNE_FFT) && h->cfg.precision == PM_FFT_FP32) {   // BF16X3 has one plan
         // Field solves only read the mass lattice: repeating them changes no state.
         if (!h->rho_valid && (st = stage_deposit(h)) != PM_OK) { h->profiling = prof; return st; }
         pmk::FftPlan best = h->plan;

@@ -209,10 +209,151 @@ __device__ __forceinline__ void stages_from(float2* buf, const float2* tw) {
 // NFFT in-place transforms of length N at buf + f*N (padded), radix RADIX
 // (4 or 8) after a leading stage for the leftover factor; tw is the LDS quarter
 // table.  Caller synchronises before; ends with a barrier.
+// ---- split-BF16 matrix-core radix-16 stages (precision variant) --------------
+// A radix-16 stage over a group of 16 butterflies is one complex 16x16 matrix
+// product X = F * x (F[m][r] = exp(-+2 pi i m r / 16)), done as the real
+// product [Xr; Xi] = [[Fr, -Fi], [Fi, Fr]] [xr; xi] on the matrix cores with
+// v_mfma_f32_16x16x16_bf16: every FP32 operand is split into bf16 hi + lo
+// parts and each real product is formed as hi*hi + hi*lo + lo*hi with FP32
+// accumulation (the lo*lo term, ~2^-16 relative, is dropped).  Operand
+// layout (16x16x16): lane l holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16]
+// for j < 4, and receives D[4(l/16)+i][l%16] for i < 4.
+typedef short v4bf __attribute__((ext_vector_type(4)));
+typedef float v4acc __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_rne(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - __uint_as_float((unsigned)(unsigned short)hi << 16));
+}
+
+// cos(2 pi j / 16), j < 16
+__constant__ float kCos16[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                                 0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
+                                 -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
+                                 0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
+
+// DFT-16 operands: [h][kh] = output half (re, im) x input half (re, im).
+struct Dft16Ops {
+    v4bf hi[2][2], lo[2][2];
+};
+
+template <bool INV>
+__device__ __forceinline__ void dft16_ops(Dft16Ops& A) {
+    const int l = threadIdx.x & 63, m = l & 15, g = l >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int idx = (m * (4 * g + j)) & 15;
+        const float c = kCos16[idx], sn = kCos16[(idx + 12) & 15];   // sin(2 pi idx/16)
+        const float fr = c, fi = INV ? sn : -sn;
+        const float v[2][2] = {{fr, -fi}, {fi, fr}};
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int kh = 0; kh < 2; ++kh) {
+                short a, b;
+                split_bf16(v[h][kh], a, b);
+                A.hi[h][kh][j] = a;
+                A.lo[h][kh][j] = b;
+            }
+    }
+}
+
+// One radix-16 Stockham stage (same indexing as stage<>), butterflies taken
+// 16 at a time per wavefront.  Results are held in registers until every
+// wavefront has read its inputs (in place, like stage<>).
+template <int N, int NFFT, int NS, bool INV>
+__device__ __forceinline__ void stage16(float2* buf, const float2* tw, const Dft16Ops& A) {
+    constexpr int R = 16, Q = N / R, NB = NFFT * Q, G = NB / 16, NW = kBlock / 64, GPW = (G + NW - 1) / NW;
+    constexpr int STEP = N / (R * NS);
+    static_assert(NB % 16 == 0, "radix-16 matrix stage needs a multiple of 16 butterflies");
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, c = l & 15, g = l >> 4;
+    v4acc dre[GPW], dim[GPW];
+    int ob[GPW];
+#pragma unroll
+    for (int q = 0; q < GPW; ++q) {
+        const int grp = w + q * NW;
+        if (G % NW == 0 || grp < G) {              // uniform per wavefront
+            const int bb = grp * 16 + c;
+            const int f = bb / Q, b = bb - f * Q, k = b & (NS - 1);
+            v4bf bhi[2], blo[2];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = 4 * g + j;
+                float2 x = buf[pad(f * N + b + r * Q)];
+                if (NS > 1) x = cmul(x, twiddle<N, INV>(tw, r * k * STEP));
+                short h0, l0, h1, l1;
+                split_bf16(x.x, h0, l0);
+                split_bf16(x.y, h1, l1);
+                bhi[0][j] = h0; blo[0][j] = l0;
+                bhi[1][j] = h1; blo[1][j] = l1;
+            }
+            v4acc acc[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                acc[h] = v4acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kh = 0; kh < 2; ++kh) {
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.lo[h][kh], bhi[kh], acc[h], 0, 0, 0);
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.hi[h][kh], blo[kh], acc[h], 0, 0, 0);
+                }
+#pragma unroll
+                for (int kh = 0; kh < 2; ++kh)
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.hi[h][kh], bhi[kh], acc[h], 0, 0, 0);
+            }
+            dre[q] = acc[0];
+            dim[q] = acc[1];
+            ob[q] = f * N + (b - k) * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GPW; ++q) {
+        const int grp = w + q * NW;
+        if (G % NW == 0 || grp < G) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) buf[pad(ob[q] + (4 * g + i) * NS)] = make_float2(dre[q][i], dim[q][i]);
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NFFT, int NS, bool INV>
+__device__ __forceinline__ void stages16_from(float2* buf, const float2* tw, const Dft16Ops& A) {
+    if constexpr (NS < N) {
+        stage16<N, NFFT, NS, INV>(buf, tw, A);
+        stages16_from<N, NFFT, NS * 16, INV>(buf, tw, A);
+    }
+}
+
+// NFFT transforms (as lds_fft) with radix-16 matrix-core stages after one
+// leading FP32 stage of radix 2, 4 or 8 for the leftover factor.
+template <int N, int NFFT, bool INV>
+__device__ __forceinline__ void lds_fft_mfma(float2* buf, const float2* tw) {
+    constexpr int L = ilog2c(N), REM = L % 4;
+    Dft16Ops A;
+    dft16_ops<INV>(A);
+    if constexpr (REM == 0) {
+        stages16_from<N, NFFT, 1, INV>(buf, tw, A);
+    } else {
+        constexpr int R0 = 1 << REM;
+        stage<N, NFFT, R0, 1, INV>(buf, tw);
+        stages16_from<N, NFFT, R0, INV>(buf, tw, A);
+    }
+}
+
+// RADIX 4 or 8: FP32 Stockham stages; RADIX 16: the split-BF16 matrix-core
+// variant above.
 template <int N, int NFFT, int RADIX, bool INV>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
     constexpr int LR = ilog2c(RADIX), L = ilog2c(N), REM = L % LR;
-    if constexpr (REM == 0) {
+    if constexpr (RADIX == 16) {
+        lds_fft_mfma<N, NFFT, INV>(buf, tw);
+    } else if constexpr (REM == 0) {
         stages_from<N, NFFT, RADIX, 1, INV>(buf, tw);
     } else {
         constexpr int R0 = 1 << REM;
@@ -466,6 +607,7 @@ hipError_t r2c_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, hip
 #define PM_R2C(RPV, RV) \
     if constexpr (rp_ok<N, RPV>()) if (p.rp == RPV && p.radix == RV) return r2c_t<N, RPV, RV>(d, count, tw, s);
     PM_R2C(1, 4) PM_R2C(2, 4) PM_R2C(4, 4) PM_R2C(1, 8) PM_R2C(2, 8) PM_R2C(4, 8)
+    if constexpr (N >= 256) if (p.rp == 1 && p.radix == 16) return r2c_t<N, 1, 16>(d, count, tw, s);
 #undef PM_R2C
     return hipErrorInvalidValue;
 }
@@ -475,6 +617,7 @@ hipError_t col_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, boo
 #define PM_COL(CBV, RV) \
     if constexpr (cb_ok<N, CBV>()) if (p.cb == CBV && p.radix == RV) return col_t<N, CBV, RV>(d, count, tw, force, s);
     PM_COL(1, 4) PM_COL(2, 4) PM_COL(4, 4) PM_COL(8, 4) PM_COL(1, 8) PM_COL(2, 8) PM_COL(4, 8) PM_COL(8, 8)
+    if constexpr (N >= 256) if (p.cb == 1 && p.radix == 16) return col_t<N, 1, 16>(d, count, tw, force, s);
 #undef PM_COL
     return hipErrorInvalidValue;
 }
@@ -484,6 +627,7 @@ hipError_t c2r_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, boo
 #define PM_C2R(RPV, RV) \
     if constexpr (rp_ok<N, RPV>()) if (p.rp == RPV && p.radix == RV) return c2r_t<N, RPV, RV>(d, count, tw, force, s);
     PM_C2R(1, 4) PM_C2R(2, 4) PM_C2R(4, 4) PM_C2R(1, 8) PM_C2R(2, 8) PM_C2R(4, 8)
+    if constexpr (N >= 256) if (p.rp == 1 && p.radix == 16) return c2r_t<N, 1, 16>(d, count, tw, force, s);
 #undef PM_C2R
     return hipErrorInvalidValue;
 }
@@ -517,6 +661,7 @@ hipError_t stencil_n(const FieldDesc* d, int count, hipStream_t s) {
     }
 
 bool plan_valid(int n, FftPlan p) {
+    if (p.radix == 16) return p.rp == 1 && p.cb == 1 && n >= 256 && n <= 8192;
     if (p.radix != 4 && p.radix != 8) return false;
     if (p.rp != 1 && p.rp != 2 && p.rp != 4) return false;
     if (p.cb != 1 && p.cb != 2 && p.cb != 4 && p.cb != 8) return false;

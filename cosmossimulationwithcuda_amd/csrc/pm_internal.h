@@ -95,7 +95,8 @@ struct FieldDesc {
     float gravity;        // force multiplier
 };
 // Tile batch and radix of the three passes: rp row pairs per row-pass block,
-// cb columns per column-pass block, radix 4 or 8.
+// cb columns per column-pass block, radix 4 or 8 (FP32 Stockham stages) or 16
+// (split-BF16 matrix-core stages, rp = cb = 1, 256 <= n <= 8192).
 struct FftPlan {
     int rp = 2, cb = 1, radix = 4;
 };

@@ -86,6 +86,7 @@ struct BatchPlan {
     std::vector<Request*> members;
     struct NGroup {
         int n;
+        pmk::FftPlan plan;
         std::vector<int> idx;
         size_t fdesc_off[2];
     };
@@ -418,9 +419,11 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members) {
     P.mgroups.clear();
     for (int i = 0; i < (int)members.size(); ++i) {
         const pm_sim* h = members[i]->sim;
-        auto ng = std::find_if(P.ngroups.begin(), P.ngroups.end(),
-                               [&](const BatchPlan::NGroup& g) { return g.n == h->cfg.n_lattice; });
-        if (ng == P.ngroups.end()) P.ngroups.push_back({h->cfg.n_lattice, {i}, {0, 0}});
+        auto ng = std::find_if(P.ngroups.begin(), P.ngroups.end(), [&](const BatchPlan::NGroup& g) {
+            return g.n == h->cfg.n_lattice && g.plan.rp == h->plan.rp && g.plan.cb == h->plan.cb &&
+                   g.plan.radix == h->plan.radix;
+        });
+        if (ng == P.ngroups.end()) P.ngroups.push_back({h->cfg.n_lattice, h->plan, {i}, {0, 0}});
         else ng->idx.push_back(i);
         auto mg = std::find_if(P.mgroups.begin(), P.mgroups.end(),
                                [&](const BatchPlan::MGroup& g) { return g.mode == h->cfg.mode; });
@@ -509,9 +512,9 @@ pm_status launch_batched_steps(pm_server* s, int steps) {
             const pm_sim* h0 = P.members[g.idx[0]]->sim;
             const auto* d = reinterpret_cast<const pmk::FieldDesc*>(db + g.fdesc_off[p]);
             const int cnt = (int)g.idx.size();
-            if (pmk::row_r2c(g.n, d, cnt, h0->tw, h0->plan, s->batch_stream) != hipSuccess ||
-                pmk::col_conv(g.n, d, cnt, h0->tw, h0->plan, true, s->batch_stream) != hipSuccess ||
-                pmk::row_c2r(g.n, d, cnt, h0->tw, h0->plan, true, s->batch_stream) != hipSuccess ||
+            if (pmk::row_r2c(g.n, d, cnt, h0->tw, g.plan, s->batch_stream) != hipSuccess ||
+                pmk::col_conv(g.n, d, cnt, h0->tw, g.plan, true, s->batch_stream) != hipSuccess ||
+                pmk::row_c2r(g.n, d, cnt, h0->tw, g.plan, true, s->batch_stream) != hipSuccess ||
                 pmk::edge_force(g.n, d, cnt, s->batch_stream) != hipSuccess)
                 return PM_EDEVICE;
         }

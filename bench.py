@@ -102,6 +102,68 @@ def copy_probe(nbytes):
     print(f"# copy probe: {nbytes} bytes device to device", file=sys.stderr, flush=True)
 
 
+# Precision variant: 4096^2 lattice, 1e8 particles, Plummer binary (a = 64
+# cells, 5e7 each, circular orbit at separation 800), CIC, dt = 0.002.
+PRECISION = {"lattice": 4096, "particles": 100_000_000, "dt": 0.002, "samples": (0, 5, 10, 20), "timed_steps": 20}
+
+
+def precision_report(seed):
+    """FP32 FFT vs split-BF16 matrix-core FFT on the same workload: throughput,
+    and at sample steps the potential / force / total-energy error of each
+    against the FP64 oracle field of that step's own deposited lattice."""
+    from cosmossimulationwithcuda_amd import ics, pm
+    from oracle import oracle as orc
+    n, npart, dt = PRECISION["lattice"], PRECISION["particles"], PRECISION["dt"]
+    t = time.perf_counter()
+    arrs = ics.plummer_binary(npart, seed, n_lattice=n)
+    print(f"# precision: ics {npart} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
+    out = {"workload": f"plummer_binary_{n}_{npart // 1_000_000}M_cic", "lattice": n, "particles": npart, "dt": dt}
+    for name, prec in (("fp32", pm.FFT_FP32), ("bf16x3", pm.FFT_BF16X3)):
+        cfg = pm.default_config(n_lattice=n, n_particles=npart, mode=pm.CIC, dt=dt, gravity=1.0, green_origin=1.0,
+                                deconv_eps=1e-3, precision=prec)
+        rows = []
+        with pm.Simulation(cfg) as s:
+            s.load(*arrs)
+            done = 0
+            for k in PRECISION["samples"]:
+                if k == 0:
+                    s.stage(pm.STAGE_DEPOSIT)
+                s.step(k - done)
+                done = k
+                s.stage(pm.STAGE_FIELD)                  # field of the current lattice
+                rho = s.read_lattice(pm.LATTICE_MASS)
+                phi = s.read_lattice(pm.LATTICE_POTENTIAL).astype(np.float64)
+                force = s.read_lattice(pm.LATTICE_FORCE).astype(np.float64)
+                shift = s.info().mass_shift
+                ke = s.diagnostics()["kinetic"]
+                phi64 = orc.potential(n, T, shift, rho)
+                fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+                mass = rho.astype(np.float64) * 2.0 ** -shift
+                e64 = ke + 0.5 * float(np.sum(mass * phi64))
+                e = ke + 0.5 * float(np.sum(mass * phi))
+                rows.append({"step": k,
+                             "potential_rel_l2": float(np.linalg.norm(phi - phi64) / np.linalg.norm(phi64)),
+                             "force_rel_l2": float(np.sqrt(np.sum((force[..., 0] - fx64) ** 2 + (force[..., 1] - fy64) ** 2)
+                                                           / np.sum(fx64 ** 2 + fy64 ** 2))),
+                             "energy_rel_err": abs(e - e64) / abs(e64)})
+                print(f"# precision {name} step {k}: {rows[-1]}", file=sys.stderr, flush=True)
+            s.profile(True)
+            s.profile_reset()
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(PRECISION["timed_steps"])
+            s.sync()
+            el = time.perf_counter() - t0
+            prof = s.profile_read()
+        fft_ms = sum(prof[k][0] / max(prof[k][1], 1) for k in ("row_r2c", "col_conv", "row_c2r"))
+        out[name] = {"particle_steps_per_s": npart * PRECISION["timed_steps"] / el,
+                     "ms_per_step": el / PRECISION["timed_steps"] * 1e3, "fft_ms_per_step": round(fft_ms, 4),
+                     "kernels_ms": {k: [round(v[0] / v[1], 4), v[1]] for k, v in prof.items() if v[1]},
+                     "error_vs_fp64": rows}
+    return out
+
+
 # Serving ensemble: independent requests arriving as a Poisson stream.
 SERVING = {"requests": 512, "rate_per_s": 40.0, "lattices": (256, 512, 1024, 2048), "particles": (1e4, 1e7),
            "steps": 200, "budget_bytes": 200 << 30, "dt": 1.0, "template": 10_000_000}
@@ -215,6 +277,8 @@ def main():
                          "kernel of known size for rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--serving", choices=("on", "off", "only"), default="on",
                     help="serving ensemble leg (plain vs batched server, saturated and Poisson arrivals)")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only the 4096^2 Plummer-binary FP32 vs split-BF16 FFT comparison (error vs FP64 per step)")
     ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
     ap.add_argument("--serving-modes", default="plain,batched")
     ap.add_argument("--serving-arrivals", default="saturated,poisson")
@@ -227,6 +291,9 @@ def main():
 
     from cosmossimulationwithcuda_amd import pm
 
+    if args.precision_report:
+        print(json.dumps({"precision": precision_report(args.seed)}), flush=True)
+        return
     if args.serving == "only":
         print(json.dumps({"serving": serving_bench(args.serving_requests, args.seed, args.serving_modes.split(","),
                                    args.serving_arrivals.split(","))}), flush=True)

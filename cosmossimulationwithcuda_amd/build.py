@@ -30,7 +30,7 @@ FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["pm_api.cpp", "pm_serve.cpp", "pm_binning.hip", "pm_fft.hip", "pm_particles.hip", "pm_spectra.hip", "pm_util.hip"]
+SOURCES = ["pm_api.cpp", "pm_serve.cpp", "pm_binning.hip", "pm_fft.hip", "pm_frame.hip", "pm_particles.hip", "pm_spectra.hip", "pm_util.hip"]
 
 
 def _compile(src):
@@ -63,5 +63,26 @@ def build(verbose=False):
     return OUT
 
 
+def build_checks():
+    """Compiles the C++ programs that exercise the public headers
+    (tests/cpp/*.cpp -> tests/bin/), linked against libpm.so."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(ROOT, "tests", "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        out = os.path.join(out_dir, f[:-4])
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(src_dir, f),
+               "-L" + PKG, "-lpm", "-Wl,-rpath,$ORIGIN/../../cosmossimulationwithcuda_amd", "-pthread", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {f}:\n{r.stderr}")
+        outs.append(out)
+    return outs
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_checks())

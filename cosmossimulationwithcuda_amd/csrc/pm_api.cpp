@@ -113,6 +113,10 @@ void release(pm_sim* h) {
     for (auto e : h->pool) (void)hipEventDestroy(e);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
+    if (h->mig_event) (void)hipEventDestroy(h->mig_event);
+    if (h->mig_host) (void)hipHostFree(h->mig_host);
+    h->mig_event = nullptr;
+    h->mig_host = nullptr;
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     h->own_stream = nullptr;
     h->stream = nullptr;
@@ -242,8 +246,44 @@ pm_status stage_push(pm_sim* h) {
 
 // Step tail with the next deposit fused into the push; the mass lattice of
 // the new positions is left in rho_buf[rho_cur] for the next field solve.
+bool capturing(const pm_sim* h) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(h->stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone;
+}
+
+// Reads the migrant count copied after an earlier push if it has landed (no
+// wait) and re-sorts now when the per-step rate is above the threshold.
+// For large simulations the host waits once the copy is two pushes old, so
+// the decision is never staler than that (a step there is far longer than
+// the launch latency, so the device queue does not drain).
+pm_status adaptive_resort(pm_sim* h) {
+    if (!h->mig_inflight) return PM_OK;
+    if (h->np >= (1 << 20) && h->pushes - h->pushes_at_copy >= 2) {
+        PM_HIP(h, hipEventSynchronize(h->mig_event));
+    } else if (hipEventQuery(h->mig_event) != hipSuccess) {
+        (void)hipGetLastError();      // "not ready" is not an error for the launches that follow
+        return PM_OK;
+    }
+    h->mig_inflight = false;
+    const unsigned long long v = *h->mig_host;
+    const int64_t steps = h->pushes_at_copy - h->pushes_at_read;
+    const double per_step = steps > 0 ? (double)(v - h->mig_last) / (double)steps : 0.0;
+    h->mig_last = v;
+    h->pushes_at_read = h->pushes_at_copy;
+    if (h->steps_since_sort > 0 && per_step * 1000.0 > (double)h->resort_permille * (double)h->np) {
+        h->adaptive_sorts += 1;
+        return sort_now(h, false);
+    }
+    return PM_OK;
+}
+
 pm_status stage_push_deposit(pm_sim* h) {
     const int n = h->cfg.n_lattice;
+    const bool adaptive = h->resort_permille > 0 && h->mig_host && !capturing(h);
+    if (adaptive) {
+        pm_status st = adaptive_resort(h);
+        if (st != PM_OK) return st;
+    }
     const bool due = h->steps_since_sort + 1 >= h->resort_interval;
     {
         KernelTimer kt(h, PM_K_CLEAR);
@@ -259,6 +299,14 @@ pm_status stage_push_deposit(pm_sim* h) {
     h->rho_cur ^= 1;
     h->rho_valid = true;
     h->steps_since_sort += 1;
+    h->pushes += 1;
+    if (adaptive && !h->mig_inflight) {
+        PM_HIP(h, hipMemcpyAsync(h->mig_host, h->migrants, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 h->stream));
+        PM_HIP(h, hipEventRecord(h->mig_event, h->stream));
+        h->mig_inflight = true;
+        h->pushes_at_copy = h->pushes;
+    }
     if (due) return sort_now(h, true);
     return PM_OK;
 }
@@ -471,6 +519,11 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
     if ((st = dalloc(h, &p, L.force)) != PM_OK) return bail(st); h->force = static_cast<float2*>(p);
     if ((st = dalloc(h, &p, L.diag)) != PM_OK) return bail(st); h->diag = static_cast<double*>(p);
 
+    if (hipHostMalloc(reinterpret_cast<void**>(&h->mig_host), sizeof(unsigned long long), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&h->mig_event, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(h, PM_ENOMEM, "pinned migrant counter"));
+    *h->mig_host = 0;
     if ((st = upload_tables(h)) != PM_OK) return bail(st);
     if ((st = build_spectra(h, T_src)) != PM_OK) return bail(st);
     // deterministic initial lattices (force/potential read before any step are zero)
@@ -607,6 +660,33 @@ pm_status pm_read_lattice(pm_handle h, int32_t which, void* dst, size_t bytes) {
     return PM_OK;
 }
 
+pm_status pm_frame(pm_handle h, int32_t blur_r, int32_t reset_extremes, float* frame) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (!frame) return fail(h, PM_EINVAL, "null frame buffer");
+    if (blur_r < 1 || blur_r > 15 || (blur_r & 1) == 0) return fail(h, PM_EINVAL, "blur_r must be odd in [1, 15]");
+    PM_HIP(h, hipSetDevice(h->cfg.device));
+    const int n = h->cfg.n_lattice;
+    const size_t cells = (size_t)n * n;
+    if (!h->frame) {
+        void* p = nullptr;
+        pm_status st = dalloc(h, &p, align256(4 * (int64_t)cells) + 256);
+        if (st != PM_OK) return st;
+        h->frame = static_cast<float*>(p);
+    }
+    if (!h->rho_valid) {
+        pm_status st = stage_deposit(h);
+        if (st != PM_OK) return st;
+    }
+    auto* tail = reinterpret_cast<char*>(h->frame) + align256(4 * (int64_t)cells);
+    PM_HIP(h, pmk::frame(n, h->rho_buf[h->rho_cur], std::ldexp(1.0f, -h->shift), blur_r,
+                         reset_extremes != 0 || !h->frame_started, h->frame, reinterpret_cast<unsigned*>(tail),
+                         reinterpret_cast<float*>(tail + 16), h->stream));
+    h->frame_started = true;
+    PM_HIP(h, hipMemcpyAsync(frame, h->frame, 4 * cells, hipMemcpyDeviceToHost, h->stream));
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
 pm_status pm_diagnostics(pm_handle h, double out[8]) {
     if (!h || !out) return fail(h, PM_EINVAL, "null argument");
     PM_HIP(h, hipSetDevice(h->cfg.device));
@@ -649,6 +729,11 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         h->resort_interval = (int)value;
         return PM_OK;
     }
+    if (k == "resort_migrants_permille") {
+        if (value < 0 || value > 1000) return fail(h, PM_EINVAL, "resort_migrants_permille must be in [0, 1000]");
+        h->resort_permille = (int)value;
+        return PM_OK;
+    }
     if (k == "deposit_chunk") {
         if (value < pmk::kDepositBlock || value > (1 << 24) || (value % pmk::kDepositBlock) != 0)
             return fail(h, PM_EINVAL, "deposit_chunk must be a multiple of 512 in [512, 2^24]");
@@ -685,6 +770,8 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
     if (!h || !name || !value) return fail(h, PM_EINVAL, "null argument");
     const std::string k(name);
     if (k == "resort_interval") *value = h->resort_interval;
+    else if (k == "resort_migrants_permille") *value = h->resort_permille;
+    else if (k == "adaptive_sorts") *value = h->adaptive_sorts;
     else if (k == "deposit_chunk") *value = h->bins.chunk;
     else if (k == "tile_log") *value = h->bins.tl;
     else if (k == "fft_rp") *value = h->plan.rp;

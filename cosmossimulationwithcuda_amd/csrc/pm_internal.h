@@ -146,6 +146,12 @@ hipError_t build_transfer(int n, int mode, double green_origin, double eps, doub
 hipError_t push(int n, int mode, const float2* force, float* x, float* y, float* vx, float* vy,
                 int64_t np, float dt, int wrap_mode, hipStream_t s);
 
+// ----- density frames (pm_frame.hip) --------------------------------------------------
+// out: N*N floats; keys: 2 unsigned; smooth: 2 floats (running extremes, kept
+// between frames).  blur_r odd in [1, 15].
+hipError_t frame(int n, const int64_t* rho, float inv_scale, int blur_r, bool reset, float* out, unsigned* keys,
+                 float* smooth, hipStream_t s);
+
 // ----- utilities (pm_util.hip) ------------------------------------------------------
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
 // Per-block FP64 partial sums: out[b*8 + {0..7}] = {mass, px, py, ke, max|v|, nonfinite, n_inb, 0}

@@ -284,6 +284,8 @@ bool admit_one(pm_server* s) {
     }
     r->sim = h;
     h->stream = s->io_stream;
+    // rounds are replayed from graphs with a fixed re-sort cadence
+    h->resort_permille = s->cfg.use_graphs ? 0 : 10;
     pm_config c = r->cfg;
     c.n_particles = h->cap;
     c.device = s->cfg.device;
@@ -607,7 +609,8 @@ pm_status run_large_round(pm_server* s, Request* r, int steps) {
             }
         }
     }
-    if (r->graph[0] && r->graph[1] && h->steps_since_sort == 0) {
+    if (r->graph[0] && r->graph[1] && h->steps_since_sort == 0 && h->cur.x == r->gstate[r->gphase].cur.x &&
+        h->rho_cur == r->gstate[r->gphase].rho_cur) {
         const int k = r->gphase;
         if (hipGraphLaunch(r->graph[k], h->stream) != hipSuccess) return PM_EDEVICE;
         const int64_t sorts = h->sorts;

@@ -32,6 +32,17 @@ struct pm_sim {
     int max_chunks = 0;
     int resort_interval = 8;
     int steps_since_sort = 0;
+    // adaptive re-sort: the migrant counter is copied to pinned host memory
+    // after a push and read without blocking at a later push; when more than
+    // resort_permille / 1000 of the particles left their tile windows per
+    // step, the particles are re-sorted before the next push
+    int resort_permille = 10;
+    unsigned long long* mig_host = nullptr;
+    hipEvent_t mig_event = nullptr;
+    bool mig_inflight = false;
+    unsigned long long mig_last = 0;
+    int64_t pushes = 0, pushes_at_copy = 0, pushes_at_read = 0;
+    int64_t adaptive_sorts = 0;
     int64_t sorts = 0;
     unsigned long long* migrants = nullptr;
     int64_t* rho_buf[2] = {nullptr, nullptr};
@@ -47,6 +58,8 @@ struct pm_sim {
     float* phi = nullptr;
     float2* force = nullptr;
     double* diag = nullptr;
+    float* frame = nullptr;                // density frame work area (allocated by the first pm_frame)
+    bool frame_started = false;
     int64_t bytes = 0;
     std::vector<void*> allocs;
     std::string err;

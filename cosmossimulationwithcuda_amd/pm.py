@@ -92,6 +92,7 @@ def lib():
         L.pm_read_lattice.argtypes = [_H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
         L.pm_diagnostics.argtypes = [_H, ctypes.POINTER(ctypes.c_double)]
         L.pm_get_info.argtypes = [_H, ctypes.POINTER(Info)]
+        L.pm_frame.argtypes = [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
         L.pm_set_param.argtypes = [_H, ctypes.c_char_p, ctypes.c_int64]
         L.pm_get_param.argtypes = [_H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.pm_autotune.argtypes = [_H, ctypes.c_int32, ctypes.c_int32]
@@ -237,6 +238,13 @@ class Simulation:
 
     def sync(self):
         _check(lib().pm_sync(self._h), self._h)
+
+    def frame(self, blur_r=3, reset=False):
+        """Density frame of the current mass lattice, float32 [N, N] (index [y, x])."""
+        n = self.cfg.n_lattice
+        out = np.empty((n, n), np.float32)
+        _check(lib().pm_frame(self._h, int(blur_r), 1 if reset else 0, out.ctypes.data), self._h)
+        return out
 
     def diagnostics(self):
         out = (ctypes.c_double * 8)()

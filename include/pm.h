@@ -125,7 +125,9 @@ pm_status pm_load_verlet(pm_handle h, const float* x, const float* y, const floa
  * the worker loop body (cuh:1336-1341): deposit, FFT convolution, gradient,
  * gather + update.  The deposit of each step after the first is fused into
  * the previous step's gather + update; the mass lattice of the positions a
- * step ends with is therefore already built when it returns. */
+ * step ends with is therefore already built when it returns.  With the
+ * adaptive re-sort on ("resort_migrants_permille" > 0) a simulation of 2^20
+ * or more particles keeps at most two steps queued ahead of the device. */
 pm_status pm_step(pm_handle h, int32_t nsteps);
 
 /* Enqueue one stage of a step (PM_STAGE_*), for stage-level checks. */
@@ -151,12 +153,26 @@ pm_status pm_diagnostics(pm_handle h, double out[8]);
 
 pm_status pm_get_info(pm_handle h, pm_info* info);
 
+/* Density frame of the current mass lattice, replacing the reference's render
+ * stage (cuh:613-828, scheduled at 1249-1256): mass per cell -> blur_r x blur_r
+ * filter with weights 1/(r+1) for r < (blur_r-1)/2 and 0 otherwise (the
+ * identity for the reference's BLUR_R = 3), zero outside the lattice -> the
+ * frame's min and max -> running extremes kept by the handle since its first
+ * frame (reset_extremes != 0 restarts them) -> 1.8 * ((v - min)/(max - min))^0.25.
+ * blur_r odd in [1, 15]; frame: N*N floats on the host, index i + j*N.
+ * Blocking.  The reference renders the lattice deposited at the start of its
+ * K-th step; this renders the lattice of the current positions. */
+pm_status pm_frame(pm_handle h, int32_t blur_r, int32_t reset_extremes, float* frame);
+
 /* Runtime tunables: "resort_interval"
- * (steps between particle re-binnings), "deposit_chunk" (particles per
+ * (steps between particle re-binnings), "resort_migrants_permille" (re-bin
+ * early when more than this many per thousand particles per step land outside
+ * their tile's window; 0 = off, default 10), "deposit_chunk" (particles per
  * particle-kernel block, a multiple of 512), "tile_log" (bin edge 2^5 or
  * 2^6 cells; not below the lattice's default) change speed, never results;
  * the FFT plan "fft_rp", "fft_cb", "fft_radix" changes FP32 rounding only
- * (see pm_autotune).  No counterpart in the reference, whose
+ * (see pm_autotune).  pm_get_param also reads "adaptive_sorts" (early
+ * re-bins so far).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
 pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);

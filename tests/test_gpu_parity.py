@@ -407,3 +407,29 @@ def test_autotune_keeps_state():
     d = np.hypot(outs[2][0] - outs[0][0], outs[2][1] - outs[0][1])
     print(f"FFT plan retuned: max |dx| after 5 steps {d.max():.2e} cells")
     assert d.max() < 1e-3
+
+
+def test_adaptive_resort_keeps_results():
+    """Fast particles leave their tile windows; the adaptive re-sort (migrant
+    rate above resort_migrants_permille) fires and changes no result bit."""
+    n, k = 512, 200000
+    x, y, _, _, m = ics.uniform_square(k, 128, 384, seed=21)
+    rng = np.random.default_rng(22)
+    vx = rng.normal(0.0, 150.0, k).astype(np.float32)      # ~1.5 cells per step at dt 0.01
+    vy = rng.normal(0.0, 150.0, k).astype(np.float32)
+    out = []
+    for permille in (0, 10):
+        with make_sim(n, k, pm.CIC) as s:
+            s.set_param("resort_interval", 64)
+            s.set_param("resort_migrants_permille", permille)
+            s.load(x, y, vx, vy, m)
+            for i in range(40):
+                s.step(1)
+                if i % 4 == 3:
+                    s.sync()
+            out.append((s.read(), s.get_param("adaptive_sorts")))
+    (a, sorts_off), (b, sorts_on) = out
+    print("adaptive sorts:", sorts_off, sorts_on)
+    assert sorts_off == 0 and sorts_on > 0
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v, equal_nan=True)

@@ -103,8 +103,10 @@ def copy_probe(nbytes):
 
 
 # Precision variant: 4096^2 lattice, 1e8 particles, Plummer binary (a = 64
-# cells, 5e7 each, circular orbit at separation 800), CIC, dt = 0.002.
-PRECISION = {"lattice": 4096, "particles": 100_000_000, "dt": 0.002, "samples": (0, 5, 10, 20), "timed_steps": 20}
+# cells, 5e7 each, circular orbit at separation 800), CIC.  dt = 5e-4: the
+# spheres start cold and collapse; their dynamical time sqrt(a^3 / (G M)) is
+# 0.072, and at dt = 2e-3 the collapsed cores move several cells per step.
+PRECISION = {"lattice": 4096, "particles": 100_000_000, "dt": 0.0005, "samples": (0, 5, 10, 20), "timed_steps": 20}
 
 
 def precision_report(seed):

@@ -729,6 +729,12 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         h->resort_interval = (int)value;
         return PM_OK;
     }
+    if (k == "bin_max_chunks") {
+        if (value < 0 || value > (1 << 20)) return fail(h, PM_EINVAL, "bin_max_chunks must be in [0, 2^20] (0 = no cap)");
+        h->bins.max_chunks_per_bin = (int)value;
+        PM_HIP(h, hipSetDevice(h->cfg.device));
+        return sort_now(h, false);
+    }
     if (k == "resort_migrants_permille") {
         if (value < 0 || value > 1000) return fail(h, PM_EINVAL, "resort_migrants_permille must be in [0, 1000]");
         h->resort_permille = (int)value;
@@ -771,6 +777,7 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
     const std::string k(name);
     if (k == "resort_interval") *value = h->resort_interval;
     else if (k == "resort_migrants_permille") *value = h->resort_permille;
+    else if (k == "bin_max_chunks") *value = h->bins.max_chunks_per_bin;
     else if (k == "adaptive_sorts") *value = h->adaptive_sorts;
     else if (k == "deposit_chunk") *value = h->bins.chunk;
     else if (k == "tile_log") *value = h->bins.tl;
@@ -971,6 +978,143 @@ const char* pm_status_string(pm_status s) {
         case PM_ENODEV: return "no device";
     }
     return "unknown status";
+}
+
+}  // extern "C"
+
+// ---- checkpoints -----------------------------------------------------------------
+namespace {
+
+struct CkptHeader {
+    char magic[8];
+    uint32_t header_bytes, version;
+    pm_config cfg;
+    int32_t mass_shift, fft_rp, fft_cb, fft_radix;
+    int32_t resort_interval, resort_permille, deposit_chunk, tile_log;
+    int64_t n_particles, steps;
+    uint64_t checksum;
+};
+constexpr char kCkptMagic[8] = {'P', 'M', 'C', 'K', 'P', 'T', 0, 1};
+
+uint64_t fnv1a(const unsigned char* p, size_t n, uint64_t hsh = 1469598103934665603ull) {
+    for (size_t i = 0; i < n; ++i) hsh = (hsh ^ p[i]) * 1099511628211ull;
+    return hsh;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pm_checkpoint_bytes(pm_handle h) {
+    if (!h) return -1;
+    return (int64_t)sizeof(CkptHeader) + 20 * h->np;
+}
+
+pm_status pm_checkpoint_write(pm_handle h, void* dst, int64_t bytes) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (!dst || bytes < pm_checkpoint_bytes(h)) return fail(h, PM_EINVAL, "checkpoint buffer too small");
+    const int64_t n = h->np;
+    auto* out = static_cast<unsigned char*>(dst);
+    float* x = reinterpret_cast<float*>(out + sizeof(CkptHeader));
+    float *y = x + n, *vx = y + n, *vy = vx + n, *m = vy + n;
+    pm_status st = pm_read(h, x, y, vx, vy);
+    if (st != PM_OK) return st;
+    if (n) {   // masses in load order: the slots' load indices scatter the sorted masses back
+        std::vector<uint32_t> id((size_t)n);
+        std::vector<float> ms((size_t)n);
+        PM_HIP(h, hipMemcpyAsync(id.data(), h->cur.id, 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+        PM_HIP(h, hipMemcpyAsync(ms.data(), h->cur.m, 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+        PM_HIP(h, hipStreamSynchronize(h->stream));
+        for (int64_t k = 0; k < n; ++k) m[id[(size_t)k]] = ms[(size_t)k];
+    }
+    CkptHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    std::memcpy(hd.magic, kCkptMagic, 8);
+    hd.header_bytes = sizeof(CkptHeader);
+    hd.version = 1;
+    hd.cfg = h->cfg;
+    hd.mass_shift = h->shift;
+    hd.fft_rp = h->plan.rp;
+    hd.fft_cb = h->plan.cb;
+    hd.fft_radix = h->plan.radix;
+    hd.resort_interval = h->resort_interval;
+    hd.resort_permille = h->resort_permille;
+    hd.deposit_chunk = h->bins.chunk;
+    hd.tile_log = h->bins.tl;
+    hd.n_particles = n;
+    hd.steps = h->pushes;
+    hd.checksum = fnv1a(out + sizeof(CkptHeader), 20 * (size_t)n);
+    std::memcpy(out, &hd, sizeof(hd));
+    return PM_OK;
+}
+
+pm_status pm_checkpoint_restore(const void* src, int64_t bytes, int32_t device, pm_handle* out) {
+    if (!src || !out) return fail(nullptr, PM_EINVAL, "null argument");
+    *out = nullptr;
+    if (bytes < (int64_t)sizeof(CkptHeader)) return fail(nullptr, PM_EINVAL, "checkpoint truncated");
+    CkptHeader hd;
+    std::memcpy(&hd, src, sizeof(hd));
+    if (std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != 1 || hd.header_bytes != sizeof(CkptHeader))
+        return fail(nullptr, PM_EINVAL, "not a checkpoint of this format");
+    const int64_t n = hd.n_particles;
+    if (n < 0 || bytes < (int64_t)sizeof(CkptHeader) + 20 * n) return fail(nullptr, PM_EINVAL, "checkpoint truncated");
+    const auto* in = static_cast<const unsigned char*>(src);
+    if (fnv1a(in + sizeof(CkptHeader), 20 * (size_t)n) != hd.checksum)
+        return fail(nullptr, PM_EINVAL, "checkpoint checksum mismatch");
+    pm_config c = hd.cfg;
+    c.device = device;
+    c.mass_shift = hd.mass_shift;            // the same fixed point as when saved
+    c.n_particles = std::max<int64_t>(c.n_particles, n);
+    pm_sim* h = nullptr;
+    pm_status st = pm_create(&c, &h);
+    if (st != PM_OK) return st;
+    h->plan.rp = hd.fft_rp;
+    h->plan.cb = hd.fft_cb;
+    h->plan.radix = hd.fft_radix;
+    if (!pmk::plan_valid(c.n_lattice, h->plan)) h->plan = plan_for(c);
+    h->resort_interval = hd.resort_interval > 0 ? hd.resort_interval : 8;
+    h->resort_permille = hd.resort_permille;
+    h->bins.chunk = hd.deposit_chunk > 0 ? hd.deposit_chunk : h->bins.chunk;
+    h->max_chunks = pmk::deposit_max_chunks(c.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
+    if (hd.tile_log != h->bins.tl && pm_set_param(h, "tile_log", hd.tile_log) != PM_OK) {
+        pm_destroy(h);
+        return fail(nullptr, PM_EINVAL, "checkpoint tile size not available");
+    }
+    const float* x = reinterpret_cast<const float*>(in + sizeof(CkptHeader));
+    st = pm_load(h, x, x + n, x + 2 * n, x + 3 * n, x + 4 * n, n);
+    if (st != PM_OK) {
+        std::string e = h->err;
+        pm_destroy(h);
+        return fail(nullptr, st, e);
+    }
+    h->pushes = hd.steps;
+    *out = h;
+    return PM_OK;
+}
+
+pm_status pm_checkpoint_save(pm_handle h, const char* path) {
+    if (!h) return fail(nullptr, PM_EINVAL, "null handle");
+    if (!path) return fail(h, PM_EINVAL, "null path");
+    std::vector<unsigned char> buf((size_t)pm_checkpoint_bytes(h));
+    pm_status st = pm_checkpoint_write(h, buf.data(), (int64_t)buf.size());
+    if (st != PM_OK) return st;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(h, PM_EINVAL, std::string("cannot open ") + path);
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if (std::fclose(f) != 0 || !ok) return fail(h, PM_EINVAL, std::string("write failed: ") + path);
+    return PM_OK;
+}
+
+pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out) {
+    if (!path || !out) return fail(nullptr, PM_EINVAL, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(nullptr, PM_EINVAL, std::string("cannot open ") + path);
+    std::vector<unsigned char> buf;
+    unsigned char tmp[1 << 16];
+    size_t got;
+    while ((got = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+    std::fclose(f);
+    return pm_checkpoint_restore(buf.data(), (int64_t)buf.size(), device, out);
 }
 
 }  // extern "C"

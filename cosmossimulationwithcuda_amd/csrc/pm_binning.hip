@@ -32,14 +32,21 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(const float* __restrict__ 
 }
 
 // One block: exclusive scans of the bin counts (-> start[0..nb]) and of the
-// deposit chunk counts of the in-bounds bins (-> chunk_start[0..nb-1]).
+// deposit chunk counts of the in-bounds bins (-> chunk_start[0..nb-1]); a
+// bin's chunks split it into equal parts.
 // Zeroes the scatter cursors.  add_parked: the counts came from the fused
 // push, which never visits the parked range; add that range's standing size
 // (from the previous start[]) to the parked bin first.
 constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count, int nb, int chunk, int add_parked,
-                                                          int64_t* __restrict__ start, int* __restrict__ chunk_start,
-                                                          int* __restrict__ cursor) {
+// Chunks of an in-bounds bin: ceil(count / chunk), at most kmax.
+__device__ __forceinline__ int chunks_of(int count, int chunk, int kmax) {
+    const int k = (int)(((int64_t)count + chunk - 1) / chunk);
+    return k < kmax ? k : kmax;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count, int nb, int chunk, int kmax,
+                                                          int add_parked, int64_t* __restrict__ start,
+                                                          int* __restrict__ chunk_start, int* __restrict__ cursor) {
     __shared__ int64_t s_cnt[kScanBlock];
     __shared__ int s_chk[kScanBlock];
     const int t = threadIdx.x;
@@ -51,7 +58,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count
     int k = 0;
     for (int b = b0; b < b1; ++b) {
         c += count[b];
-        if (b < nb - 1) k += (int)(((int64_t)count[b] + chunk - 1) / chunk);
+        if (b < nb - 1) k += chunks_of(count[b], chunk, kmax);
     }
     s_cnt[t] = c;
     s_chk[t] = k;
@@ -71,7 +78,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count
         chunk_start[b] = runk;
         cursor[b] = 0;
         run += count[b];
-        if (b < nb - 1) runk += (int)(((int64_t)count[b] + chunk - 1) / chunk);
+        if (b < nb - 1) runk += chunks_of(count[b], chunk, kmax);
     }
     if (t == kScanBlock - 1) start[nb] = s_cnt[t];
 }
@@ -203,7 +210,8 @@ hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t
         hipLaunchKernelGGL(k_bin_count, dim3(grid_for(np, 64, 2048)), dim3(kBlock), sizeof(int) * nb, s, cur.x, cur.y,
                            np, n, bs.tl, bs.count);
     }
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk, counted ? 1 : 0, bs.start,
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk,
+                       bs.max_chunks_per_bin > 0 ? bs.max_chunks_per_bin : (1 << 30), counted ? 1 : 0, bs.start,
                        bs.chunk_start, bs.cursor);
     if (np > 0) {
         const int64_t blocks = (np + kScatPerBlock - 1) / kScatPerBlock;

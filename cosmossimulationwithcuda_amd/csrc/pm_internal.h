@@ -37,13 +37,16 @@ struct ParticleArrays {
     uint32_t* id;   // load index of the particle in each slot
 };
 
+constexpr int kMaxChunksPerBinDefault = 64;
 struct BinState {
     int* count;          // [nbins + 1] particles per bin (last = parked)
     int64_t* start;      // [nbins + 2] exclusive prefix of count
     int* chunk_start;    // [nbins + 1] exclusive prefix of deposit chunks per in-bounds bin
     int* cursor;         // [nbins + 1] scatter cursors
-    int chunk;           // particles per deposit block
+    int chunk;           // particles per deposit block (target)
     int tl;              // log2 of the tile edge in cells
+    int max_chunks_per_bin = kMaxChunksPerBinDefault;   // a dense bin is split into at most this many
+                                                        // equal chunks (fewer window flushes to the same cells)
 };
 
 // Lattice sizes with compiled kernels (powers of two).

@@ -118,9 +118,10 @@ __device__ __forceinline__ void deposit_global(unsigned long long* __restrict__ 
 }
 
 // Block c -> (bin b, slots [s0, s1)): chunk_start is the exclusive prefix of
-// chunks over the in-bounds bins, chunk_start[nin] their total.
+// chunks over the in-bounds bins, chunk_start[nin] their total; the k chunks
+// of a bin of P particles split it into equal parts [j P/k, (j+1) P/k).
 __device__ __forceinline__ bool chunk_of(const int* __restrict__ chunk_start, const int64_t* __restrict__ start,
-                                         int nin, int chunk, int c, int& b, int64_t& s0, int64_t& s1) {
+                                         int nin, int c, int& b, int64_t& s0, int64_t& s1) {
     if (c >= chunk_start[nin]) return false;
     int lo = 0, hi = nin;
     while (lo < hi) {
@@ -128,8 +129,9 @@ __device__ __forceinline__ bool chunk_of(const int* __restrict__ chunk_start, co
         if (chunk_start[mid] <= c) lo = mid + 1; else hi = mid;
     }
     b = lo - 1;
-    s0 = start[b] + (int64_t)(c - chunk_start[b]) * chunk;
-    s1 = min(s0 + chunk, start[b + 1]);
+    const int64_t p = start[b + 1] - start[b], k = chunk_start[b + 1] - chunk_start[b], j = c - chunk_start[b];
+    s0 = start[b] + j * p / k;
+    s1 = start[b] + (j + 1) * p / k;
     return true;
 }
 
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit_bin(const float* __re
     const int tpr = n >> TL;
     int b;
     int64_t s0, s1;
-    if (!chunk_of(chunk_start, start, tpr * tpr, chunk, blockIdx.x, b, s0, s1)) return;
+    if (!chunk_of(chunk_start, start, tpr * tpr, blockIdx.x, b, s0, s1)) return;
     const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
     for (int e = threadIdx.x; e < W * W; e += kDepositBlock) win[e] = 0ull;
     if (threadIdx.x == 0) s_mig = 0;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     const int tpr = n >> TL, nin = tpr * tpr;
     int b;
     int64_t s0, s1;
-    if (!chunk_of(chunk_start, start, nin, chunk, blockIdx.x, b, s0, s1)) return;
+    if (!chunk_of(chunk_start, start, nin, blockIdx.x, b, s0, s1)) return;
     const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
     for (int e = threadIdx.x; e < W * W; e += BLK) {
         win[e] = 0ull;

@@ -93,6 +93,12 @@ def lib():
         L.pm_diagnostics.argtypes = [_H, ctypes.POINTER(ctypes.c_double)]
         L.pm_get_info.argtypes = [_H, ctypes.POINTER(Info)]
         L.pm_frame.argtypes = [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+        L.pm_checkpoint_bytes.argtypes = [_H]
+        L.pm_checkpoint_bytes.restype = ctypes.c_int64
+        L.pm_checkpoint_write.argtypes = [_H, ctypes.c_void_p, ctypes.c_int64]
+        L.pm_checkpoint_restore.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_H)]
+        L.pm_checkpoint_save.argtypes = [_H, ctypes.c_char_p]
+        L.pm_checkpoint_open.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(_H)]
         L.pm_set_param.argtypes = [_H, ctypes.c_char_p, ctypes.c_int64]
         L.pm_get_param.argtypes = [_H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.pm_autotune.argtypes = [_H, ctypes.c_int32, ctypes.c_int32]
@@ -170,6 +176,27 @@ class Simulation:
     @property
     def handle(self):
         return self._h
+
+    @classmethod
+    def _adopt(cls, h):
+        self = cls.__new__(cls)
+        self._h = h
+        info = Info()
+        _check(lib().pm_get_info(h, ctypes.byref(info)), h)
+        self.cfg = default_config(n_lattice=info.n_lattice, n_particles=info.capacity, device=info.device)
+        self.n = info.n_lattice
+        return self
+
+    # checkpoints
+    def checkpoint(self):
+        """The state as checkpoint bytes (header + float32 x, y, vx, vy, m)."""
+        nb = lib().pm_checkpoint_bytes(self._h)
+        buf = np.empty(nb, np.uint8)
+        _check(lib().pm_checkpoint_write(self._h, buf.ctypes.data, nb), self._h)
+        return buf.tobytes()
+
+    def save(self, path):
+        _check(lib().pm_checkpoint_save(self._h, os.fsencode(path)), self._h)
 
     # state
     def load(self, x, y, vx, vy, m):
@@ -267,6 +294,21 @@ class Simulation:
             _check(lib().pm_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(cnt)), self._h)
             res[name] = (ms.value, cnt.value)
         return res
+
+
+def restore(data, device=0):
+    """A Simulation restored from checkpoint bytes."""
+    buf = np.frombuffer(data, np.uint8)
+    h = _H()
+    _check(lib().pm_checkpoint_restore(buf.ctypes.data, buf.size, int(device), ctypes.byref(h)))
+    return Simulation._adopt(h)
+
+
+def open_checkpoint(path, device=0):
+    """A Simulation restored from a checkpoint file."""
+    h = _H()
+    _check(lib().pm_checkpoint_open(os.fsencode(path), int(device), ctypes.byref(h)))
+    return Simulation._adopt(h)
 
 
 # ----- serving engine (include/pm_serve.h) -----------------------------------------

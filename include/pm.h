@@ -164,11 +164,24 @@ pm_status pm_get_info(pm_handle h, pm_info* info);
  * K-th step; this renders the lattice of the current positions. */
 pm_status pm_frame(pm_handle h, int32_t blur_r, int32_t reset_extremes, float* frame);
 
+/* Checkpoints (no counterpart in the reference, whose state never leaves the
+ * device): a header (configuration, fixed-point shift, FFT plan, tunables,
+ * step count, FNV-1a checksum) followed by x, y, vx, vy, m as little-endian
+ * float32 in load order.  Restoring creates a handle on `device` whose next
+ * steps are bit-identical to those of the saved handle.  pm_checkpoint_bytes
+ * gives the buffer size for pm_checkpoint_write (-1 for a null handle). */
+int64_t pm_checkpoint_bytes(pm_handle h);
+pm_status pm_checkpoint_write(pm_handle h, void* dst, int64_t bytes);
+pm_status pm_checkpoint_restore(const void* src, int64_t bytes, int32_t device, pm_handle* out);
+pm_status pm_checkpoint_save(pm_handle h, const char* path);
+pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
+
 /* Runtime tunables: "resort_interval"
  * (steps between particle re-binnings), "resort_migrants_permille" (re-bin
  * early when more than this many per thousand particles per step land outside
  * their tile's window; 0 = off, default 10), "deposit_chunk" (particles per
- * particle-kernel block, a multiple of 512), "tile_log" (bin edge 2^5 or
+ * particle-kernel block, a multiple of 512), "bin_max_chunks" (a dense tile
+ * is split into at most this many equal blocks, default 64, 0 = no cap), "tile_log" (bin edge 2^5 or
  * 2^6 cells; not below the lattice's default) change speed, never results;
  * the FFT plan "fft_rp", "fft_cb", "fft_radix" changes FP32 rounding only
  * (see pm_autotune).  pm_get_param also reads "adaptive_sorts" (early

@@ -111,3 +111,12 @@ def test_server_without_device_reports_enodev():
     assert pm.lib().pm_server_create(ctypes.byref(c), ctypes.byref(h)) == pm.PM_ENODEV
     with pytest.raises(pm.PMError):
         pm.Server(c)
+
+
+def test_checkpoint_restore_validates_before_device():
+    h = ctypes.c_void_p()
+    junk = (ctypes.c_ubyte * 4096)()
+    assert pm.lib().pm_checkpoint_restore(junk, 4096, 0, ctypes.byref(h)) == pm.PM_EINVAL
+    assert b"not a checkpoint" in pm.lib().pm_last_error(None)
+    assert pm.lib().pm_checkpoint_restore(junk, 16, 0, ctypes.byref(h)) == pm.PM_EINVAL
+    assert pm.lib().pm_checkpoint_bytes(None) == -1

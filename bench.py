@@ -52,6 +52,8 @@ def algorithmic_bytes(kernel, n, np_):
         "sort": 0,                                # re-binning is implementation traffic, not algorithmic
         # fused push + next deposit: read x, y, vx, vy, m; write x, y, vx, vy; read force; write int64 lattice
         "push_deposit": 36 * np_ + 16 * n * n,
+        # the same step with the re-sort fused in (moving m and the load index is sort traffic)
+        "push_deposit_sort": 36 * np_ + 16 * n * n,
     }[kernel]
 
 

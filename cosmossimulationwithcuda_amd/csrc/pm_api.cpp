@@ -92,7 +92,7 @@ Layout layout_of(const pm_config* c) {
     const int64_t dblocks = std::max(pmk::diag_blocks(cap), pmk::diag_lattice_blocks((int)n));
     L.diag = align256(8 * 8 * dblocks);
     L.fdesc = align256(2 * sizeof(pmk::FieldDesc));
-    L.total = 12 * L.particles + L.bins + 2 * L.rho + L.fdesc + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force +
+    L.total = 12 * L.particles + 2 * L.bins + 2 * L.rho + L.fdesc + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force +
               L.diag;
     L.setup_work = 16 * n * n;
     return L;
@@ -181,6 +181,7 @@ pm_status upload_field_desc(pm_sim* h) {
 
 // ---- stages -----------------------------------------------------------------
 pm_status sort_now(pm_sim* h, bool counted) {
+    h->scatter_pending = false;
     KernelTimer kt(h, PM_K_SORT);
     PM_HIP(h, pmk::sort_particles(h->cfg.n_lattice, h->cur, h->alt, h->np, h->bins, counted, h->stream));
     std::swap(h->cur, h->alt);
@@ -236,6 +237,7 @@ pm_status stage_field(pm_sim* h) {
 }
 
 pm_status stage_push(pm_sim* h) {
+    h->scatter_pending = false;          // positions change: the counted bins no longer apply
     KernelTimer kt(h, PM_K_PUSH);
     PM_HIP(h, pmk::push(h->cfg.n_lattice, h->cfg.mode, h->force, h->cur.x, h->cur.y, h->cur.vx, h->cur.vy, h->np,
                         h->cfg.dt, h->cfg.wrap_mode, h->stream));
@@ -284,15 +286,49 @@ pm_status stage_push_deposit(pm_sim* h) {
         pm_status st = adaptive_resort(h);
         if (st != PM_OK) return st;
     }
-    const bool due = h->steps_since_sort + 1 >= h->resort_interval;
+    // Fused re-sort (interval >= 3): the step where steps_since_sort + 2 reaches
+    // the interval counts the new bins into the second bin set; the next step
+    // writes the particles into the other buffers in that order, so the cycle
+    // is `interval` steps with no separate sort pass.  Otherwise the count step
+    // is followed by the separate counting sort.
+    const bool fused = h->fused_sort && h->resort_interval >= 3 && pmk::scatter_fits(n, h->bins.tl);
+    if (h->scatter_pending && fused) {
+        {
+            KernelTimer kt(h, PM_K_CLEAR);
+            PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur ^ 1], (size_t)8 * n * n, h->stream));
+        }
+        {
+            KernelTimer kt(h, PM_K_PUSH_DEPOSIT_SORT);
+            PM_HIP(h, pmk::push_deposit_scatter(n, h->cfg.mode, h->cfg.wrap_mode, h->force, h->cur, h->alt, h->bins,
+                                                next_bins(h), h->max_chunks, h->cfg.dt, std::ldexp(1.0f, h->shift),
+                                                h->rho_buf[h->rho_cur ^ 1], h->migrants, h->stream));
+        }
+        std::swap(h->cur, h->alt);
+        std::swap(h->bins.count, h->nx_count);
+        std::swap(h->bins.start, h->nx_start);
+        std::swap(h->bins.chunk_start, h->nx_chunk_start);
+        std::swap(h->bins.cursor, h->nx_cursor);
+        h->scatter_pending = false;
+        h->rho_cur ^= 1;
+        h->rho_valid = true;
+        h->steps_since_sort = 0;
+        h->sorts += 1;
+        h->pushes += 1;
+        return PM_OK;
+    }
+    h->scatter_pending = false;
+    const bool due = h->steps_since_sort + (fused ? 2 : 1) >= h->resort_interval;
+    int* count = fused ? h->nx_count : h->bins.count;
     {
         KernelTimer kt(h, PM_K_CLEAR);
         PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur ^ 1], (size_t)8 * n * n, h->stream));
-        if (due) PM_HIP(h, pmk::fill_zero(h->bins.count, sizeof(int) * (size_t)h->nbins, h->stream));
+        if (due) PM_HIP(h, pmk::fill_zero(count, sizeof(int) * (size_t)h->nbins, h->stream));
     }
     {
         KernelTimer kt(h, PM_K_PUSH_DEPOSIT);
-        PM_HIP(h, pmk::push_deposit(n, h->cfg.mode, h->cfg.wrap_mode, h->force, h->cur, h->bins, h->max_chunks,
+        pmk::BinState bs = h->bins;
+        bs.count = count;
+        PM_HIP(h, pmk::push_deposit(n, h->cfg.mode, h->cfg.wrap_mode, h->force, h->cur, bs, h->max_chunks,
                                     h->cfg.dt, std::ldexp(1.0f, h->shift), h->rho_buf[h->rho_cur ^ 1], h->migrants,
                                     due, h->stream));
     }
@@ -300,6 +336,11 @@ pm_status stage_push_deposit(pm_sim* h) {
     h->rho_valid = true;
     h->steps_since_sort += 1;
     h->pushes += 1;
+    if (due && fused) {
+        KernelTimer kt(h, PM_K_SORT);
+        PM_HIP(h, pmk::scan_next(n, next_bins(h), h->bins.start, h->stream));
+        h->scatter_pending = true;
+    }
     if (adaptive && !h->mig_inflight) {
         PM_HIP(h, hipMemcpyAsync(h->mig_host, h->migrants, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                  h->stream));
@@ -307,7 +348,7 @@ pm_status stage_push_deposit(pm_sim* h) {
         h->mig_inflight = true;
         h->pushes_at_copy = h->pushes;
     }
-    if (due) return sort_now(h, true);
+    if (due && !fused) return sort_now(h, true);
     return PM_OK;
 }
 
@@ -402,17 +443,33 @@ pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src) {
     return upload_field_desc(h);
 }
 
+pmk::BinState next_bins(const pm_sim* h) {
+    pmk::BinState b = h->bins;
+    b.count = h->nx_count;
+    b.start = h->nx_start;
+    b.chunk_start = h->nx_chunk_start;
+    b.cursor = h->nx_cursor;
+    return b;
+}
+
 HostState save_state(const pm_sim* h) {
-    return HostState{h->cur, h->alt, h->rho_cur, h->field_parity, h->steps_since_sort, h->rho_valid, h->sorts};
+    return HostState{h->cur,          h->alt,         h->bins,           next_bins(h),        h->rho_cur,
+                     h->field_parity, h->steps_since_sort, h->rho_valid, h->scatter_pending, h->sorts};
 }
 
 void restore_state(pm_sim* h, const HostState& s) {
     h->cur = s.cur;
     h->alt = s.alt;
+    h->bins = s.bins;
+    h->nx_count = s.next.count;
+    h->nx_start = s.next.start;
+    h->nx_chunk_start = s.next.chunk_start;
+    h->nx_cursor = s.next.cursor;
     h->rho_cur = s.rho_cur;
     h->field_parity = s.field_parity;
     h->steps_since_sort = s.steps_since_sort;
     h->rho_valid = s.rho_valid;
+    h->scatter_pending = s.scatter_pending;
     h->sorts = s.sorts;
 }
 
@@ -502,6 +559,16 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
         h->bins.cursor = reinterpret_cast<int*>(bp + 2 * ib);
         h->bins.start = reinterpret_cast<int64_t*>(bp + 3 * ib);
         h->migrants = reinterpret_cast<unsigned long long*>(bp + 3 * ib + align256(8 * (int64_t)(h->nbins + 2)));
+    }
+    if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
+    {
+        char* bp = static_cast<char*>(p);
+        const int64_t ib = align256(4 * (int64_t)(h->nbins + 1));
+        h->nx_count = reinterpret_cast<int*>(bp);
+        h->nx_chunk_start = reinterpret_cast<int*>(bp + ib);
+        h->nx_cursor = reinterpret_cast<int*>(bp + 2 * ib);
+        h->nx_start = reinterpret_cast<int64_t*>(bp + 3 * ib);
+        if (pmk::fill_zero(p, (size_t)L.bins, h->stream) != hipSuccess) return bail(fail(h, PM_EDEVICE, "clear"));
     }
     // both mass lattices in one block: 2 * 8 N^2 bytes, also the FP64 setup work area
     if ((st = dalloc(h, &p, 2 * L.rho)) != PM_OK) return bail(st);
@@ -729,6 +796,12 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         h->resort_interval = (int)value;
         return PM_OK;
     }
+    if (k == "fused_sort") {
+        if (value != 0 && value != 1) return fail(h, PM_EINVAL, "fused_sort must be 0 or 1");
+        h->fused_sort = (int)value;
+        PM_HIP(h, hipSetDevice(h->cfg.device));
+        return sort_now(h, false);
+    }
     if (k == "bin_max_chunks") {
         if (value < 0 || value > (1 << 20)) return fail(h, PM_EINVAL, "bin_max_chunks must be in [0, 2^20] (0 = no cap)");
         h->bins.max_chunks_per_bin = (int)value;
@@ -778,6 +851,7 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
     if (k == "resort_interval") *value = h->resort_interval;
     else if (k == "resort_migrants_permille") *value = h->resort_permille;
     else if (k == "bin_max_chunks") *value = h->bins.max_chunks_per_bin;
+    else if (k == "fused_sort") *value = h->fused_sort;
     else if (k == "adaptive_sorts") *value = h->adaptive_sorts;
     else if (k == "deposit_chunk") *value = h->bins.chunk;
     else if (k == "tile_log") *value = h->bins.tl;

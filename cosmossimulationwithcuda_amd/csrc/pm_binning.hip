@@ -44,13 +44,21 @@ __device__ __forceinline__ int chunks_of(int count, int chunk, int kmax) {
     return k < kmax ? k : kmax;
 }
 
+// old_start: the previous starts (may alias start); standing_cursor: start
+// the parked bin's cursor after the standing particles (fused re-sort, which
+// moves them separately).
 __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count, int nb, int chunk, int kmax,
-                                                          int add_parked, int64_t* __restrict__ start,
+                                                          int add_parked, int standing_cursor,
+                                                          const int64_t* old_start, int64_t* start,
                                                           int* __restrict__ chunk_start, int* __restrict__ cursor) {
     __shared__ int64_t s_cnt[kScanBlock];
     __shared__ int s_chk[kScanBlock];
+    __shared__ int s_standing;
     const int t = threadIdx.x;
-    if (add_parked && t == 0) count[nb - 1] += (int)(start[nb] - start[nb - 1]);
+    if (t == 0) {
+        s_standing = add_parked ? (int)(old_start[nb] - old_start[nb - 1]) : 0;
+        if (add_parked) count[nb - 1] += s_standing;
+    }
     __syncthreads();
     const int per = (nb + kScanBlock - 1) / kScanBlock;
     const int b0 = t * per, b1 = min(nb, b0 + per);
@@ -76,7 +84,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count
     for (int b = b0; b < b1; ++b) {
         start[b] = run;
         chunk_start[b] = runk;
-        cursor[b] = 0;
+        cursor[b] = (standing_cursor && b == nb - 1) ? s_standing : 0;
         run += count[b];
         if (b < nb - 1) runk += chunks_of(count[b], chunk, kmax);
     }
@@ -211,13 +219,21 @@ hipError_t sort_particles(int n, ParticleArrays cur, ParticleArrays alt, int64_t
                            np, n, bs.tl, bs.count);
     }
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, bs.count, nb, bs.chunk,
-                       bs.max_chunks_per_bin > 0 ? bs.max_chunks_per_bin : (1 << 30), counted ? 1 : 0, bs.start,
-                       bs.chunk_start, bs.cursor);
+                       bs.max_chunks_per_bin > 0 ? bs.max_chunks_per_bin : (1 << 30), counted ? 1 : 0, 0,
+                       static_cast<const int64_t*>(bs.start), bs.start, bs.chunk_start, bs.cursor);
     if (np > 0) {
         const int64_t blocks = (np + kScatPerBlock - 1) / kScatPerBlock;
         hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(kScatBlock), 2 * sizeof(int) * nb, s, cur, alt, np,
                            n, bs.tl, bs.start, bs.cursor);
     }
+    return hipGetLastError();
+}
+
+hipError_t scan_next(int n, BinState next, const int64_t* old_start, hipStream_t s) {
+    const int nb = bin_count_of(n, next.tl);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, next.count, nb, next.chunk,
+                       next.max_chunks_per_bin > 0 ? next.max_chunks_per_bin : (1 << 30), 1, 1, old_start, next.start,
+                       next.chunk_start, next.cursor);
     return hipGetLastError();
 }
 

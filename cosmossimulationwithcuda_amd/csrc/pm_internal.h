@@ -72,6 +72,18 @@ hipError_t unpermute(const uint32_t* id, int64_t np, const float* a0, const floa
 // max_chunks: upper bound of the chunk count (grid size).  migrants counts the
 // contributions made outside the tile window.
 int deposit_max_chunks(int n, int tl, int64_t capacity, int chunk);
+// Re-sort fused into a step: push + deposit the chunks of `cur` (read from
+// src), writing each particle to dst grouped by the bin of its start
+// position (next.start / next.cursor from scan_next), and move the standing
+// parked particles.  scatter_fits: the kernel's LDS fits this lattice/tile.
+bool scatter_fits(int n, int tl);
+hipError_t push_deposit_scatter(int n, int mode, int wrap_mode, const float2* force, ParticleArrays src,
+                                ParticleArrays dst, BinState cur, BinState next, int max_chunks, float dt,
+                                float scale, int64_t* rho_next, unsigned long long* migrants, hipStream_t s);
+// next.count (counted by a push with count_bins) -> next.start, chunk_start,
+// cursor; the parked bin also holds the particles standing in cur's parked
+// range (old_start), and its cursor starts after them.
+hipError_t scan_next(int n, BinState next, const int64_t* old_start, hipStream_t s);
 hipError_t deposit(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
                    float scale, int64_t* rho, unsigned long long* migrants, hipStream_t s);
 // Fused step tail: gather + kick-drift every particle of every in-bounds bin,

@@ -335,6 +335,171 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }
 
+// --------------------------------------------- fused step kernel with re-sort --
+// The re-sort fused into a step: the chunk's particles (read from `src` in the
+// order of the current chunk map) are pushed and deposited as above, then
+// written -- with mass and load index -- to `dst`, grouped by the bin of
+// their position at the start of the step: slot = next_start[bin] + the
+// block's base (one global atomic per block, pass and bin, taken by the
+// lane of rank 0) + the lane's rank in LDS.  The bins were counted during the
+// previous step, so `dst` ends in the order of this step's start positions,
+// one step stale, well inside the kHalo margin of the tile windows.
+template <int TL, int MODE, int WRAP>
+__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
+    const float2* __restrict__ force, ParticleArrays src, ParticleArrays dst, int n, float dt,
+    const int64_t* __restrict__ start, const int* __restrict__ chunk_start, const int64_t* __restrict__ next_start,
+    int* __restrict__ next_cursor, float scale, unsigned long long* __restrict__ rho,
+    unsigned long long* __restrict__ migrants) {
+    constexpr int W = TileGeom<TL>::kWin, E = TileGeom<TL>::kEdge, BLK = fused_block<TL>();
+    __shared__ unsigned long long win[W * W];
+    __shared__ float2 fwin[W * W];
+    __shared__ unsigned int s_mig;
+    extern __shared__ int lds[];
+    const int tpr = n >> TL, nin = tpr * tpr;
+    int* lcnt = lds;             // [nin + 1]
+    int* lbase = lds + nin + 1;  // [nin + 1]
+    int b;
+    int64_t s0, s1;
+    if (!chunk_of(chunk_start, start, nin, blockIdx.x, b, s0, s1)) return;
+    const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
+    for (int e = threadIdx.x; e < W * W; e += BLK) {
+        win[e] = 0ull;
+        const int gx = wx0 + e % W, gy = wy0 + e / W;
+        fwin[e] = ((unsigned)gx < (unsigned)n && (unsigned)gy < (unsigned)n) ? force[(size_t)gy * n + gx]
+                                                                             : make_float2(0.f, 0.f);
+    }
+    for (int e = threadIdx.x; e <= nin; e += BLK) lcnt[e] = 0;
+    if (threadIdx.x == 0) s_mig = 0;
+    __syncthreads();
+    const float fhi = (float)(n - 1);
+    const int64_t q0 = s0 >> 2, q1 = (s1 + 3) >> 2;
+    const int nq = (int)(q1 - q0);
+    const v4f* X = reinterpret_cast<const v4f*>(src.x) + q0;
+    const v4f* Y = reinterpret_cast<const v4f*>(src.y) + q0;
+    const v4f* VX = reinterpret_cast<const v4f*>(src.vx) + q0;
+    const v4f* VY = reinterpret_cast<const v4f*>(src.vy) + q0;
+    const v4f* M = reinterpret_cast<const v4f*>(src.m) + q0;
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const v4u* ID = reinterpret_cast<const v4u*>(src.id) + q0;
+    const int lo_k = (int)(s0 - 4 * q0);
+    const int hi_k = (int)(s1 - 4 * (q1 - 1));
+    unsigned int mig = 0;
+    constexpr int U = 2;
+    for (int base = 0; base < nq; base += U * BLK) {     // same trip count in every thread (barriers inside)
+        float px[U][4], py[U][4], ux[U][4], uy[U][4], pm[U][4];
+        unsigned pid[U][4];
+        int bn[U][4], rk[U][4];
+        bool ok[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = base + u * BLK + threadIdx.x;
+            v4f a = {-1.f, -1.f, -1.f, -1.f}, bb = a, c = {0.f, 0.f, 0.f, 0.f}, d = c, e = c;
+            v4u id = {0u, 0u, 0u, 0u};
+            if (j < nq) {
+                a = __builtin_nontemporal_load(X + j);
+                bb = __builtin_nontemporal_load(Y + j);
+                c = __builtin_nontemporal_load(VX + j);
+                d = __builtin_nontemporal_load(VY + j);
+                e = __builtin_nontemporal_load(M + j);
+                id = __builtin_nontemporal_load(ID + j);
+            }
+            px[u][0] = a.x; px[u][1] = a.y; px[u][2] = a.z; px[u][3] = a.w;
+            py[u][0] = bb.x; py[u][1] = bb.y; py[u][2] = bb.z; py[u][3] = bb.w;
+            ux[u][0] = c.x; ux[u][1] = c.y; ux[u][2] = c.z; ux[u][3] = c.w;
+            uy[u][0] = d.x; uy[u][1] = d.y; uy[u][2] = d.z; uy[u][3] = d.w;
+            pm[u][0] = e.x; pm[u][1] = e.y; pm[u][2] = e.z; pm[u][3] = e.w;
+            pid[u][0] = id.x; pid[u][1] = id.y; pid[u][2] = id.z; pid[u][3] = id.w;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ok[u][k] = j < nq && (j > 0 || k >= lo_k) && (j < nq - 1 || k < hi_k);
+                bn[u][k] = -1;
+                if (ok[u][k]) {
+                    bn[u][k] = bin_of(px[u][k], py[u][k], n, TL);    // bin of the start position
+                    rk[u][k] = atomicAdd(&lcnt[bn[u][k]], 1);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!ok[u][k] || !in_bounds(px[u][k], py[u][k], fhi)) continue;
+                const int xi = (int)px[u][k], yi = (int)py[u][k];
+                const int lx = xi - wx0, ly = yi - wy0;
+                const float fx = px[u][k] - (float)xi, fy = py[u][k] - (float)yi;
+                float2 a;
+                if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
+                    a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
+                else
+                    a = gather<MODE>(force + (size_t)yi * n + xi, n, fx, fy);
+                kick_drift<WRAP>(a, n, dt, px[u][k], py[u][k], ux[u][k], uy[u][k]);
+            }
+        }
+        bool spill[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                spill[u][k] = ok[u][k] && in_bounds(px[u][k], py[u][k], fhi) &&
+                              !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (spill[u][k]) { deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale); ++mig; }
+        __syncthreads();                                   // ranks of this pass complete
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (bn[u][k] >= 0 && rk[u][k] == 0) lbase[bn[u][k]] = atomicAdd(&next_cursor[bn[u][k]], lcnt[bn[u][k]]);
+        __syncthreads();                                   // bases known
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (bn[u][k] < 0) continue;
+                const int64_t sl = next_start[bn[u][k]] + lbase[bn[u][k]] + rk[u][k];
+                dst.x[sl] = px[u][k];
+                dst.y[sl] = py[u][k];
+                dst.vx[sl] = ux[u][k];
+                dst.vy[sl] = uy[u][k];
+                dst.m[sl] = pm[u][k];
+                dst.id[sl] = pid[u][k];
+            }
+        __syncthreads();                                   // counts read: reset for the next pass
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (bn[u][k] >= 0 && rk[u][k] == 0) lcnt[bn[u][k]] = 0;
+        __syncthreads();
+    }
+    if (mig) atomicAdd(&s_mig, mig);
+    __syncthreads();
+    flush_window<W, BLK>(win, wx0, wy0, rho, n);
+    if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
+}
+
+// Particles standing in the parked bin (never visited by the step kernels)
+// move to the front of the parked range of `dst`; the scan started that
+// bin's cursor after them.
+__global__ __launch_bounds__(kBlock) void k_copy_parked(ParticleArrays src, ParticleArrays dst,
+                                                         const int64_t* __restrict__ old_start,
+                                                         const int64_t* __restrict__ new_start, int parked) {
+    const int64_t a = old_start[parked], cnt = old_start[parked + 1] - a, d = new_start[parked];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * kBlock) {
+        dst.x[d + i] = src.x[a + i];
+        dst.y[d + i] = src.y[a + i];
+        dst.vx[d + i] = src.vx[a + i];
+        dst.vy[d + i] = src.vy[a + i];
+        dst.m[d + i] = src.m[a + i];
+        dst.id[d + i] = src.id[a + i];
+    }
+}
+
 // ----------------------------------------------------------- plain push --
 // Four consecutive particles per lane: 16-byte nontemporal loads and stores
 // of x, y, vx, vy; force from the global lattice.  Used by the single-stage
@@ -523,6 +688,50 @@ hipError_t push_deposit_direct(int mode, const PartDesc* d, const int* block_sta
         hipLaunchKernelGGL((k_push_deposit_direct<0>), dim3(total_blocks), dim3(kBlock), 0, s, d, block_start, count);
     else
         hipLaunchKernelGGL((k_push_deposit_direct<1>), dim3(total_blocks), dim3(kBlock), 0, s, d, block_start, count);
+    return hipGetLastError();
+}
+
+template <int TL, int MODE, int WRAP>
+hipError_t scatter_t(int n, const float2* force, ParticleArrays src, ParticleArrays dst, BinState cur,
+                     BinState next, int max_chunks, float dt, float scale, unsigned long long* r,
+                     unsigned long long* migrants, hipStream_t s) {
+    const size_t lds = 2 * sizeof(int) * (size_t)(((n >> TL) * (n >> TL)) + 1);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_scatter<TL, MODE, WRAP>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_push_deposit_scatter<TL, MODE, WRAP>), dim3(max_chunks), dim3(fused_block<TL>()), lds, s,
+                       force, src, dst, n, dt, cur.start, cur.chunk_start, next.start, next.cursor, scale, r,
+                       migrants);
+    return hipGetLastError();
+}
+
+template <int TL>
+hipError_t scatter_mode(int n, int mode, int wrap, const float2* force, ParticleArrays src, ParticleArrays dst,
+                        BinState cur, BinState next, int max_chunks, float dt, float scale, unsigned long long* r,
+                        unsigned long long* migrants, hipStream_t s) {
+    if (mode == 0 && wrap == 0) return scatter_t<TL, 0, 0>(n, force, src, dst, cur, next, max_chunks, dt, scale, r, migrants, s);
+    if (mode == 0) return scatter_t<TL, 0, 1>(n, force, src, dst, cur, next, max_chunks, dt, scale, r, migrants, s);
+    if (wrap == 0) return scatter_t<TL, 1, 0>(n, force, src, dst, cur, next, max_chunks, dt, scale, r, migrants, s);
+    return scatter_t<TL, 1, 1>(n, force, src, dst, cur, next, max_chunks, dt, scale, r, migrants, s);
+}
+
+bool scatter_fits(int n, int tl) {
+    constexpr size_t kLds = 160 * 1024;
+    const size_t bins = (size_t)(n >> tl) * (n >> tl) + 1;
+    const size_t w = tl == 5 ? TileGeom<5>::kWin : TileGeom<6>::kWin;
+    return 2 * sizeof(int) * bins + 16 * w * w + 64 <= kLds;
+}
+
+hipError_t push_deposit_scatter(int n, int mode, int wrap_mode, const float2* force, ParticleArrays src,
+                                ParticleArrays dst, BinState cur, BinState next, int max_chunks, float dt,
+                                float scale, int64_t* rho_next, unsigned long long* migrants, hipStream_t s) {
+    auto* r = reinterpret_cast<unsigned long long*>(rho_next);
+    hipError_t e;
+    if (cur.tl == 5) e = scatter_mode<5>(n, mode, wrap_mode, force, src, dst, cur, next, max_chunks, dt, scale, r, migrants, s);
+    else e = scatter_mode<6>(n, mode, wrap_mode, force, src, dst, cur, next, max_chunks, dt, scale, r, migrants, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_copy_parked, dim3(64), dim3(kBlock), 0, s, src, dst, cur.start, next.start,
+                       (n >> cur.tl) * (n >> cur.tl));
     return hipGetLastError();
 }
 

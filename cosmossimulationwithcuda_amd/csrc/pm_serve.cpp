@@ -535,6 +535,7 @@ pm_status launch_batched_steps(pm_server* s, int steps) {
 // The batched kernels ignore particle order, so the tile sort is not run; a
 // later tiled step (the request's tail) counts bins and re-sorts right away.
 void advance_batched_state(pm_sim* h, int steps) {
+    h->scatter_pending = false;
     h->rho_cur ^= (steps & 1);
     h->rho_valid = true;
     h->field_parity = h->rho_cur ^ 1;
@@ -585,7 +586,7 @@ pm_status run_large_round(pm_server* s, Request* r, int steps) {
     r->rounds_done += 1;
     if (s->cfg.use_graphs && !r->graphs_tried && r->rounds_done > 1) {
         r->graphs_tried = true;
-        if (h->steps_since_sort == 0 && h->resort_interval == steps && !h->profiling) {
+        if (h->steps_since_sort == 0 && !h->scatter_pending && h->resort_interval == steps && !h->profiling) {
             const pmi::HostState s0 = pmi::save_state(h);
             bool ok = true;
             for (int k = 0; k < 2 && ok; ++k) {
@@ -609,7 +610,8 @@ pm_status run_large_round(pm_server* s, Request* r, int steps) {
             }
         }
     }
-    if (r->graph[0] && r->graph[1] && h->steps_since_sort == 0 && h->cur.x == r->gstate[r->gphase].cur.x &&
+    if (r->graph[0] && r->graph[1] && h->steps_since_sort == 0 && !h->scatter_pending &&
+        h->cur.x == r->gstate[r->gphase].cur.x &&
         h->rho_cur == r->gstate[r->gphase].rho_cur) {
         const int k = r->gphase;
         if (hipGraphLaunch(r->graph[k], h->stream) != hipSuccess) return PM_EDEVICE;

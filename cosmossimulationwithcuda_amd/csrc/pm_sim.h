@@ -27,6 +27,14 @@ struct pm_sim {
     int shift = 40;
     pmk::ParticleArrays cur{}, alt{};      // sorted state and the counting-sort target / read scratch
     pmk::BinState bins{};
+    // second bin set: filled by the counting step of a fused re-sort while
+    // `bins` still maps the current order; swapped in after the re-sort step
+    int* nx_count = nullptr;
+    int64_t* nx_start = nullptr;
+    int* nx_chunk_start = nullptr;
+    int* nx_cursor = nullptr;
+    bool scatter_pending = false;          // the next push writes the re-sorted order
+    int fused_sort = 1;                    // re-sort inside the step kernel when it fits
     int nbins = 0;                         // in-bounds bins + parked bin
     int nbins_alloc = 0;                   // bins the bin arrays were sized for
     int max_chunks = 0;
@@ -101,10 +109,13 @@ pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src);
 // Host-side state that pm_step advances (for replaying captured step graphs).
 struct HostState {
     pmk::ParticleArrays cur, alt;
+    pmk::BinState bins, next;
     int rho_cur, field_parity, steps_since_sort;
-    bool rho_valid;
+    bool rho_valid, scatter_pending;
     int64_t sorts;
 };
+// The bin set the counting step of a fused re-sort fills (parameters of `bins`).
+pmk::BinState next_bins(const pm_sim* h);
 HostState save_state(const pm_sim* h);
 void restore_state(pm_sim* h, const HostState& s);
 

@@ -58,7 +58,8 @@ enum { PM_STAGE_DEPOSIT = 0, PM_STAGE_FIELD = 1, PM_STAGE_PUSH = 2 };
 /* timed kernels (pm_profile_read) */
 enum {
     PM_K_CLEAR = 0, PM_K_DEPOSIT = 1, PM_K_ROW_R2C = 2, PM_K_COL_CONV = 3, PM_K_ROW_C2R = 4,
-    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_SORT = 8, PM_K_PUSH_DEPOSIT = 9, PM_K_COUNT = 10
+    PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_SORT = 8, PM_K_PUSH_DEPOSIT = 9,
+    PM_K_PUSH_DEPOSIT_SORT = 10, PM_K_COUNT = 11
 };
 
 /* Replaces the compile-time namespace Constants (cuh:29-74, main.cu:5-25) and

@@ -239,7 +239,8 @@ def serving_run(pm, ics, reqs, templates, mode, poisson):
             "peak_GB": round(st["peak_bytes"] / 1e9, 2), "rounds": st["rounds"],
             "graph_launches": st["graph_launches"], "batched_rounds": st["batched_launch_rounds"],
             "pool_hits": st["pool_hits"], "graph_captures": st["graph_captures"], "failed": st["failed"],
-            "worker_s": {k: round(st[k], 3) for k in ("admit_s", "launch_s", "readback_s", "wait_s")}}
+            "worker_s": {k: round(st[k], 3) for k in ("admit_s", "launch_s", "readback_s", "wait_s", "create_s",
+                                                     "upload_s")}}
 
 
 def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson")):

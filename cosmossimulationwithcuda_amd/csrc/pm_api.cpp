@@ -512,7 +512,7 @@ int pm_device_count(void) {
 
 namespace pmi {
 
-pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
+pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStream_t work_stream) {
     if (!out) return fail(nullptr, PM_EINVAL, "null output handle");
     *out = nullptr;
     std::string why;
@@ -534,7 +534,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
         h->own_stream = nullptr;
         return bail(fail(h, PM_EDEVICE, "hipStreamCreate failed"));
     }
-    h->stream = h->own_stream;
+    h->stream = work_stream ? work_stream : h->own_stream;
     void* p;
     float** parts[10] = {&h->cur.x, &h->cur.y, &h->cur.vx, &h->cur.vy, &h->cur.m,
                          &h->alt.x, &h->alt.y, &h->alt.vx, &h->alt.vy, &h->alt.m};
@@ -610,7 +610,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src) {
 
 extern "C" {
 
-pm_status pm_create(const pm_config* cfg, pm_handle* out) { return pmi::create(cfg, out, nullptr); }
+pm_status pm_create(const pm_config* cfg, pm_handle* out) { return pmi::create(cfg, out, nullptr, nullptr); }
 
 pm_status pm_destroy(pm_handle h) {
     if (!h) return fail(nullptr, PM_EINVAL, "null handle");

@@ -127,7 +127,7 @@ struct pm_server {
     double particle_steps = 0, steps_done = 0, t_first_submit = -1, t_last_done = 0;
     int64_t submitted = 0, completed = 0, failed = 0;
     int64_t rounds = 0, batched_rounds = 0, graph_launches = 0, graph_captures = 0, pool_hits = 0;
-    double t_admit = 0, t_launch = 0, t_readback = 0, t_wait = 0;
+    double t_admit = 0, t_launch = 0, t_readback = 0, t_wait = 0, t_create = 0, t_upload = 0;
     // io thread only
     std::vector<pm_sim*> pool;
     std::map<SpectraKey, float*> spectra;            // transfer functions by physics parameters
@@ -177,13 +177,13 @@ void finish_failed(pm_server* s, Request* r, std::string msg) {   // by value: m
 }
 
 // A pooled simulation of the same lattice whose capacity holds the request
-// and is at most 4x its capacity class (the smallest such); nullptr if none.
+// and is at most 64x its capacity class (the smallest such); nullptr if none.
 pm_sim* take_pooled(pm_server* s, const Request* r) {
     const int64_t need = capacity_class(r->n);
     int best = -1;
     for (size_t i = 0; i < s->pool.size(); ++i) {
         const pm_sim* h = s->pool[i];
-        if (h->cfg.n_lattice == r->cfg.n_lattice && h->cap >= r->n && h->cap <= 4 * need &&
+        if (h->cfg.n_lattice == r->cfg.n_lattice && h->cap >= r->n && h->cap <= 64 * need &&
             (best < 0 || h->cap < s->pool[(size_t)best]->cap))
             best = (int)i;
     }
@@ -264,7 +264,13 @@ bool admit_one(pm_server* s) {
         pm_config c = r->cfg;
         c.n_particles = capacity_class(r->n);
         c.device = s->cfg.device;
-        if (pmi::create(&c, &h, cached_spectra(s, c)) != PM_OK) {
+        const double tc = now_s();
+        const pm_status cst = pmi::create(&c, &h, cached_spectra(s, c), s->io_stream);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->t_create += now_s() - tc;
+        }
+        if (cst != PM_OK) {
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->queue.pop_front();
@@ -291,9 +297,14 @@ bool admit_one(pm_server* s) {
     c.device = s->cfg.device;
     pm_status st = pmi::reconfigure(h, &c, cached_spectra(s, c));
     if (st == PM_OK) remember_spectra(s, h);
+    const double tu = now_s();
     if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);
     if (st == PM_OK) st = pmi::stage_deposit(h);
     if (st == PM_OK) st = pm_sync(h);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->t_upload += now_s() - tu;
+    }
     if (st != PM_OK) {
         finish_failed(s, r, h->err);
         std::lock_guard<std::mutex> lk(s->mu);
@@ -912,6 +923,8 @@ pm_status pm_server_get_stats(pm_server_handle s, pm_server_stats* o) {
     o->launch_s = s->t_launch;
     o->readback_s = s->t_readback;
     o->wait_s = s->t_wait;
+    o->create_s = s->t_create;
+    o->upload_s = s->t_upload;
     return PM_OK;
 }
 

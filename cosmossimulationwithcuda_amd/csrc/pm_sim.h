@@ -99,8 +99,9 @@ pm_status stage_push(pm_sim* h);
 pm_status stage_push_deposit(pm_sim* h);
 // pm_create with an optional ready transfer function (device, [N/2+1][N]
 // floats, built for the same lattice, mode, G(0) and eps) copied instead of
-// rebuilt.
-pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src);
+// rebuilt, and an optional stream for the setup work (the handle then issues
+// on it; its own stream is still created).
+pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStream_t work_stream);
 // Re-targets an existing handle (same lattice size, capacity >= its needs) to
 // another configuration: rebuilds (or copies from T_src) the transfer function
 // and the stencil symbol when the physics parameters differ.

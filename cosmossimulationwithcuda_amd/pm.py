@@ -339,7 +339,7 @@ class ServerStats(ctypes.Structure):
                                        "request_latency_p50_ms", "request_latency_p99_ms")] + [
         (k, ctypes.c_int64) for k in ("bytes_in_use", "peak_bytes", "rounds", "batched_launch_rounds",
                                       "graph_launches", "graph_captures", "pool_hits")] + [
-        (k, ctypes.c_double) for k in ("admit_s", "launch_s", "readback_s", "wait_s")]
+        (k, ctypes.c_double) for k in ("admit_s", "launch_s", "readback_s", "wait_s", "create_s", "upload_s")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -62,6 +62,8 @@ typedef struct pm_server_stats {
     /* worker-thread wall time by phase (s): admission (allocation, upload, first
      * deposit), launching rounds, reading back results, waiting for the device */
     double admit_s, launch_s, readback_s, wait_s;
+    /* admission split: creating simulations, and uploading + first deposit */
+    double create_s, upload_s;
 } pm_server_stats;
 
 void pm_server_config_default(pm_server_config* cfg);

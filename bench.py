@@ -243,6 +243,27 @@ def serving_run(pm, ics, reqs, templates, mode, poisson):
                                                      "upload_s")}}
 
 
+def serving_naive_loop(pm, ics, reqs, templates):
+    """The literal one-at-a-time loop a caller writes without the server: for
+    each request create a simulation, load, step, read back, destroy."""
+    wall = 0.0
+    steps = psteps = 0
+    for r in reqs:
+        x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
+        cfg = pm.default_config(n_lattice=r["n_lattice"], n_particles=r["n"], mode=pm.CIC, dt=SERVING["dt"],
+                                gravity=g, green_origin=1.0, deconv_eps=1e-3)
+        t0 = time.perf_counter()
+        with pm.Simulation(cfg) as s:
+            s.load(x, y, vx, vy, m)
+            s.step(SERVING["steps"])
+            s.read()
+        wall += time.perf_counter() - t0
+        steps += SERVING["steps"]
+        psteps += SERVING["steps"] * r["n"]
+    return {"wall_s": round(wall, 3), "steps_per_s": steps / wall, "particle_steps_per_s": psteps / wall,
+            "note": "time of create + load + steps + read-back + destroy per request, summed; array generation excluded"}
+
+
 def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson")):
     from cosmossimulationwithcuda_amd import ics, pm
     t = time.perf_counter()
@@ -262,9 +283,15 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
                 continue
             print(f"# serving {tag} {name}", file=sys.stderr, flush=True)
             out[f"{tag}_{name}"] = serving_run(pm, ics, reqs, templates, mode, poisson)
-        if len(modes) == 2:
+        if "plain" in modes and "batched" in modes:
             p, b = out[f"{tag}_plain"], out[f"{tag}_batched"]
             out[f"{tag}_speedup_steps_per_s"] = round(b["steps_per_s"] / p["steps_per_s"], 3)
+    if "saturated" in arrivals and "naive" in modes:
+        print("# serving naive per-request loop", file=sys.stderr, flush=True)
+        out["naive_loop"] = serving_naive_loop(pm, ics, reqs, templates)
+        if "saturated_batched" in out:
+            out["saturated_speedup_vs_naive_loop"] = round(
+                out["saturated_batched"]["steps_per_s"] / out["naive_loop"]["steps_per_s"], 3)
     return out
 
 
@@ -285,7 +312,7 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="only the 4096^2 Plummer-binary FP32 vs split-BF16 FFT comparison (error vs FP64 per step)")
     ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
-    ap.add_argument("--serving-modes", default="plain,batched")
+    ap.add_argument("--serving-modes", default="plain,batched,naive")
     ap.add_argument("--serving-arrivals", default="saturated,poisson")
     args = ap.parse_args()
 

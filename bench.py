@@ -32,6 +32,9 @@ WORKLOADS = {
     "colliding_disks_2048_100M_cic": (2048, 100_000_000, 1, 0.002, "colliding"),
     "colliding_disks_2048_100M_ngp": (2048, 100_000_000, 0, 0.002, "colliding"),
     "single_disk_2048_20M_cic": (2048, 20_000_000, 1, 0.005, "single"),
+    # full one-GPU scale: the same two disks with 5e8 particles each
+    "colliding_disks_2048_1B_cic": (2048, 1_000_000_000, 1, 0.002, "colliding_parallel"),
+    "colliding_disks_2048_1B_ngp": (2048, 1_000_000_000, 0, 0.002, "colliding_parallel"),
 }
 
 
@@ -61,6 +64,8 @@ def make_ics(gen, np_, seed):
     from cosmossimulationwithcuda_amd import ics
     if gen == "colliding":
         return ics.colliding_disks(np_, seed)
+    if gen == "colliding_parallel":
+        return ics.colliding_disks_parallel(np_, seed)
     return ics.single_disk(np_, seed)
 
 
@@ -403,10 +408,11 @@ def main():
 
     if args.cpu_steps > 0:
         threads = args.cpu_threads or os.cpu_count()
-        print(f"# cpu baseline: {args.cpu_steps} oracle steps on {threads} threads", file=sys.stderr, flush=True)
-        v, el_cpu, used = cpu_baseline(n, mode, dt, arrs, args.cpu_steps, threads)
+        cpu_steps = max(1, int(args.cpu_steps * 100_000_000 // np_))     # bounded sample: ~the same CPU time
+        print(f"# cpu baseline: {cpu_steps} oracle steps on {threads} threads", file=sys.stderr, flush=True)
+        v, el_cpu, used = cpu_baseline(n, mode, dt, arrs, cpu_steps, threads)
         out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": used, "kind": "port",
-                               "sample": f"{args.cpu_steps} full steps of the same workload ({np_} particles, "
+                               "sample": f"{cpu_steps} full steps of the same workload ({np_} particles, "
                                          f"{n}^2) after 1 warm-up step, {el_cpu:.1f}s; oracle/pm_oracle.c "
                                          f"(int64 deposit, FP64 FFT field, FP32 push), host {cpu_model()}"}
     else:

@@ -182,6 +182,7 @@ pm_status upload_field_desc(pm_sim* h) {
 // ---- stages -----------------------------------------------------------------
 pm_status sort_now(pm_sim* h, bool counted) {
     h->scatter_pending = false;
+    h->resort_request = false;
     KernelTimer kt(h, PM_K_SORT);
     PM_HIP(h, pmk::sort_particles(h->cfg.n_lattice, h->cur, h->alt, h->np, h->bins, counted, h->stream));
     std::swap(h->cur, h->alt);
@@ -254,11 +255,13 @@ bool capturing(const pm_sim* h) {
 }
 
 // Reads the migrant count copied after an earlier push if it has landed (no
-// wait) and re-sorts now when the per-step rate is above the threshold.
+// wait) and, when the per-step rate is above the threshold, re-sorts: with
+// the fused re-sort by starting its cycle at this push (count now, write the
+// new order in the next step), else with the separate sort right away.
 // For large simulations the host waits once the copy is two pushes old, so
 // the decision is never staler than that (a step there is far longer than
 // the launch latency, so the device queue does not drain).
-pm_status adaptive_resort(pm_sim* h) {
+pm_status adaptive_resort(pm_sim* h, bool fused) {
     if (!h->mig_inflight) return PM_OK;
     if (h->np >= (1 << 20) && h->pushes - h->pushes_at_copy >= 2) {
         PM_HIP(h, hipEventSynchronize(h->mig_event));
@@ -274,6 +277,10 @@ pm_status adaptive_resort(pm_sim* h) {
     h->pushes_at_read = h->pushes_at_copy;
     if (h->steps_since_sort > 0 && per_step * 1000.0 > (double)h->resort_permille * (double)h->np) {
         h->adaptive_sorts += 1;
+        if (fused) {
+            h->resort_request = true;
+            return PM_OK;
+        }
         return sort_now(h, false);
     }
     return PM_OK;
@@ -282,16 +289,17 @@ pm_status adaptive_resort(pm_sim* h) {
 pm_status stage_push_deposit(pm_sim* h) {
     const int n = h->cfg.n_lattice;
     const bool adaptive = h->resort_permille > 0 && h->mig_host && !capturing(h);
-    if (adaptive) {
-        pm_status st = adaptive_resort(h);
+    const bool fused = h->fused_sort && h->resort_interval >= 3 && pmk::scatter_fits(n, h->bins.tl);
+    if (adaptive && !(h->scatter_pending && fused)) {
+        pm_status st = adaptive_resort(h, fused);
         if (st != PM_OK) return st;
     }
     // Fused re-sort (interval >= 3): the step where steps_since_sort + 2 reaches
-    // the interval counts the new bins into the second bin set; the next step
-    // writes the particles into the other buffers in that order, so the cycle
-    // is `interval` steps with no separate sort pass.  Otherwise the count step
-    // is followed by the separate counting sort.
-    const bool fused = h->fused_sort && h->resort_interval >= 3 && pmk::scatter_fits(n, h->bins.tl);
+    // the interval (or an adaptive request is pending) counts the new bins
+    // into the second bin set; the next step writes the particles into the
+    // other buffers in that order, so the cycle is `interval` steps with no
+    // separate sort pass.  Otherwise the count step is followed by the
+    // separate counting sort.
     if (h->scatter_pending && fused) {
         {
             KernelTimer kt(h, PM_K_CLEAR);
@@ -317,7 +325,8 @@ pm_status stage_push_deposit(pm_sim* h) {
         return PM_OK;
     }
     h->scatter_pending = false;
-    const bool due = h->steps_since_sort + (fused ? 2 : 1) >= h->resort_interval;
+    const bool due = (fused && h->resort_request) || h->steps_since_sort + (fused ? 2 : 1) >= h->resort_interval;
+    h->resort_request = false;
     int* count = fused ? h->nx_count : h->bins.count;
     {
         KernelTimer kt(h, PM_K_CLEAR);

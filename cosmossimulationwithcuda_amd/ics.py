@@ -7,6 +7,8 @@ inputs.  Positions are in lattice cells, G is the force multiplier, and the
 circular speed follows the lattice force law a = G M(<r) / r^2 of the 1/r
 potential.  Returns float32 arrays (x, y, vx, vy, m).
 """
+import os
+
 import numpy as np
 
 
@@ -150,3 +152,60 @@ def ensemble_request(template, n_lattice, n, seed):
     half = np.float32(0.5)
     return (cx + s * tx[sl], cy + s * ty[sl], half * tvx[sl], half * tvy[sl], np.ones(n, np.float32),
             0.25 * float(s) / n)
+
+
+# ----- parallel generation for very large sets ---------------------------------------
+
+def _disk_chunk(args):
+    (names, off, cnt, cx, cy, rd, rmax, mtot, bulk_vx, seed, gravity) = args
+    from multiprocessing import shared_memory
+    rng = np.random.default_rng(seed)
+    r = _disk_radii(rng, cnt, rd, rmax)
+    a = rng.uniform(0.0, 2.0 * np.pi, cnt)
+    vc = np.sqrt(gravity * mtot * _enclosed_fraction(r, rd, rmax) / np.maximum(r, 1e-3))
+    ca, sa = np.cos(a), np.sin(a)
+    vals = (cx + r * ca, cy + r * sa,
+            -vc * sa + 0.02 * vc * rng.standard_normal(cnt) + bulk_vx,
+            vc * ca + 0.02 * vc * rng.standard_normal(cnt))
+    shms = [shared_memory.SharedMemory(name=nm) for nm in names]
+    try:
+        for shm, v in zip(shms[:4], vals):
+            np.ndarray((off + cnt,), np.float32, buffer=shm.buf)[off:off + cnt] = v
+        np.ndarray((off + cnt,), np.float32, buffer=shms[4].buf)[off:off + cnt] = 1.0
+    finally:
+        for shm in shms:
+            shm.close()
+    return cnt
+
+
+def colliding_disks_parallel(n, seed, workers=None, chunk=4_000_000, n_lattice=2048, rd=96.0,
+                             centres=((700.0, 1024.0), (1348.0, 1024.0)), approach=0.3, gravity=1.0):
+    """colliding_disks' law for very large n, generated in chunks by worker
+    processes (seeded per chunk, so the set depends only on n, seed and
+    chunk; it is a different draw from colliding_disks(n, seed)).  Returns
+    float32 arrays (x, y, vx, vy, m)."""
+    import multiprocessing as mp
+    from multiprocessing import shared_memory
+    workers = workers or min(16, os.cpu_count() or 1)
+    half = n // 2
+    rmax = 5.0 * rd
+    shms = [shared_memory.SharedMemory(create=True, size=max(4 * n, 4)) for _ in range(5)]
+    try:
+        names = [s.name for s in shms]
+        jobs = []
+        off = 0
+        for k, (cx, cy) in enumerate(centres):
+            nk = half if k == 0 else n - half
+            v = approach * circular_speed(nk, rd, rmax, rd, gravity)
+            for c0 in range(0, nk, chunk):
+                cnt = min(chunk, nk - c0)
+                jobs.append((names, off + c0, cnt, cx, cy, rd, rmax, float(nk), v if k == 0 else -v,
+                             (seed, k, c0 // chunk), gravity))
+            off += nk
+        with mp.get_context("spawn").Pool(workers) as pool:
+            pool.map(_disk_chunk, jobs)
+        return tuple(np.ndarray((n,), np.float32, buffer=s.buf).copy() for s in shms)
+    finally:
+        for s in shms:
+            s.close()
+            s.unlink()

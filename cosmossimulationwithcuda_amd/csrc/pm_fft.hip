@@ -470,13 +470,56 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
 template <int N, int RP, int RADIX, bool FORCE>
 __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict__ descs,
                                                      const float2* __restrict__ tw) {
-    __shared__ float2 buf[padded((FORCE ? 2 : 1) * RP * N)];
+    // MERGE: the potential row pairs and the force rows go through one
+    // transform batch (one read of the spectra, half the stage barriers).
+    // Measured faster up to N = 1024; above, the larger LDS footprint halves
+    // the blocks in flight and costs more than it saves.
+    constexpr bool MERGE = FORCE && N <= 1024 && padded(3 * RP * N) * 8 + 2 * N <= 160 * 1024;
+    __shared__ float2 buf[padded((MERGE ? 3 : (FORCE ? 2 : 1)) * RP * N)];
     __shared__ float2 twl[N / 4];
     const FieldDesc d = descs[blockIdx.y];
     constexpr int NK = N / 2 + 1;
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
     load_twiddles<N>(twl, tw);
+    if constexpr (MERGE) {
+        for (int e = t; e < NK * RP; e += kBlock) {
+            const int k = e / RP, f = e - k * RP;
+            const size_t src = (size_t)k * N + j0 + 2 * f;
+            const float4 v = *reinterpret_cast<const float4*>(d.H + src);
+            const float4 w = *reinterpret_cast<const float4*>(d.Hfy + src);
+            const bool edge = k == 0 || k == N / 2;
+            float2 xa = make_float2(v.x, v.y), xb = make_float2(v.z, v.w);
+            if (edge) { xa.y = 0.f; xb.y = 0.f; }
+            buf[pad(f * N + k)] = make_float2(xa.x - xb.y, xa.y + xb.x);
+            if (!edge) buf[pad(f * N + N - k)] = make_float2(xa.x + xb.y, xb.x - xa.y);
+            const float dv = d.dk[k];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float2 p = h ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+                float2 fy = h ? make_float2(w.z, w.w) : make_float2(w.x, w.y);
+                const float2 fx = make_float2(-p.y * dv, p.x * dv);     // i * d(kx) * Phi-hat
+                if (edge) fy.y = 0.f;
+                const int row = RP + 2 * f + h;
+                buf[pad(row * N + k)] = make_float2(fx.x - fy.y, fx.y + fy.x);
+                if (!edge) buf[pad(row * N + N - k)] = make_float2(fx.x + fy.y, fy.x - fx.y);
+            }
+        }
+        __syncthreads();
+        lds_fft<N, 3 * RP, RADIX, true>(buf, twl);
+        for (int e = t; e < RP * N; e += kBlock) {
+            const int f = e / N, i = e - f * N;
+            const size_t ra = (size_t)(j0 + 2 * f) * N + i;
+            const float2 z = buf[pad(e)];
+            d.phi[ra] = z.x;
+            d.phi[ra + N] = z.y;
+        }
+        for (int e = t; e < 2 * RP * N; e += kBlock) {
+            const int q = e / N, i = e - q * N;
+            d.force[(size_t)(j0 + q) * N + i] = buf[pad(RP * N + e)];
+        }
+        return;
+    }
     // potential: rows (j0+2f, j0+2f+1) packed as one complex row
     for (int e = t; e < NK * RP; e += kBlock) {
         const int k = e / RP, f = e - k * RP;
@@ -599,6 +642,7 @@ hipError_t c2r_t(const FieldDesc* d, int count, const float2* tw, bool force, hi
 // the same rule for the host (with the 1/32 LDS padding).
 template <int N, int RP>
 constexpr bool rp_ok() { return padded(2 * RP * N) * 8 + 2 * N <= 160 * 1024 && 2 * RP <= N; }
+// (k_row_c2r merges its two transform batches when three rows per pair fit.)
 template <int N, int CB>
 constexpr bool cb_ok() { return padded(2 * CB * N) * 8 + 2 * N <= 160 * 1024 && CB <= N / 2; }
 

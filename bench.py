@@ -269,6 +269,28 @@ def serving_naive_loop(pm, ics, reqs, templates):
             "note": "time of create + load + steps + read-back + destroy per request, summed; array generation excluded"}
 
 
+def serving_cpu_baseline(ics, reqs, templates, steps=5):
+    """The one-at-a-time loop on the host cores with the oracle's step (int64
+    deposit, FP64 FFT field, FP32 push) over a subset of the requests, a few
+    steps each (a bounded sample)."""
+    from oracle import oracle as orc
+    threads = os.cpu_count()
+    orc.set_threads(threads)
+    el = 0.0
+    psteps = 0
+    for r in reqs:
+        x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
+        t0 = time.perf_counter()
+        st = orc.MixedStepper(r["n_lattice"], 1, np.float32(SERVING["dt"]), np.float32(g), 1.0, 1e-3, 0, 30,
+                              x, y, vx, vy, m)
+        st.step(steps)
+        el += time.perf_counter() - t0
+        psteps += steps * r["n"]
+    return {"value": psteps / el, "unit": "particle-steps/s", "cores": orc.num_threads(), "kind": "port",
+            "sample": f"first {len(reqs)} requests of the ensemble, {steps} steps each (set-up included), "
+                      f"{el:.1f}s; oracle/pm_oracle.c, host {cpu_model()}"}
+
+
 def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson")):
     from cosmossimulationwithcuda_amd import ics, pm
     t = time.perf_counter()
@@ -291,6 +313,8 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
         if "plain" in modes and "batched" in modes:
             p, b = out[f"{tag}_plain"], out[f"{tag}_batched"]
             out[f"{tag}_speedup_steps_per_s"] = round(b["steps_per_s"] / p["steps_per_s"], 3)
+    if "cpu" in modes:
+        out["cpu_baseline"] = serving_cpu_baseline(ics, reqs[:16], templates)
     if "saturated" in arrivals and "naive" in modes:
         print("# serving naive per-request loop", file=sys.stderr, flush=True)
         out["naive_loop"] = serving_naive_loop(pm, ics, reqs, templates)
@@ -317,7 +341,7 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="only the 4096^2 Plummer-binary FP32 vs split-BF16 FFT comparison (error vs FP64 per step)")
     ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
-    ap.add_argument("--serving-modes", default="plain,batched,naive")
+    ap.add_argument("--serving-modes", default="plain,batched,naive,cpu")
     ap.add_argument("--serving-arrivals", default="saturated,poisson")
     args = ap.parse_args()
 

@@ -337,9 +337,10 @@ pm_status stage_push_deposit(pm_sim* h) {
         KernelTimer kt(h, PM_K_PUSH_DEPOSIT);
         pmk::BinState bs = h->bins;
         bs.count = count;
+        const bool pipelined = h->np < (int64_t)pmk::kPipelineMaxDensity * n * n;
         PM_HIP(h, pmk::push_deposit(n, h->cfg.mode, h->cfg.wrap_mode, h->force, h->cur, bs, h->max_chunks,
                                     h->cfg.dt, std::ldexp(1.0f, h->shift), h->rho_buf[h->rho_cur ^ 1], h->migrants,
-                                    due, h->stream));
+                                    due, pipelined, h->stream));
     }
     h->rho_cur ^= 1;
     h->rho_valid = true;
@@ -944,7 +945,7 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         const int tl_default = pmk::tile_log_for(n);
         for (int tl : {5, 6}) {
             if (tl < tl_default) continue;
-            for (int chunk : {8192, 16384, 32768}) {
+            for (int chunk : {8192, 12288, 16384, 32768}) {
                 for (int resort : {4, 8, 12}) {
                     h->bins.tl = tl;
                     h->nbins = pmk::bin_count_of(n, tl);

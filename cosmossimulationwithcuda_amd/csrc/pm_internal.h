@@ -14,7 +14,7 @@ constexpr int kBlock = 256;   // threads per block for every streaming kernel (4
 // for lattices up to 2048 (at most 4096 bins, 17.7 KB windows) and 64 above.
 constexpr int kHalo = 7;
 constexpr int kDepositBlock = 512;
-constexpr int kDepositChunkDefault = 16384;          // particles per particle-kernel block (tunable)
+constexpr int kDepositChunkDefault = 12288;          // particles per particle-kernel block (tunable)
 
 inline int tile_log_for(int n) { return n <= 2048 ? 5 : 6; }
 template <int TL>
@@ -89,9 +89,12 @@ hipError_t deposit(int n, int mode, const float* x, const float* y, const float*
 // Fused step tail: gather + kick-drift every particle of every in-bounds bin,
 // then deposit its new position into rho_next (zero on entry).  count_bins:
 // also histogram the new bins into bs.count (zero on entry) for the next sort.
+// pipelined: the software-pipelined variant (faster below kPipelineMaxDensity
+// particles per cell; slower above, where LDS atomics contend).
+constexpr int kPipelineMaxDensity = 64;
 hipError_t push_deposit(int n, int mode, int wrap_mode, const float2* force, ParticleArrays p, BinState bs,
                         int max_chunks, float dt, float scale, int64_t* rho_next, unsigned long long* migrants,
-                        bool count_bins, hipStream_t s);
+                        bool count_bins, bool pipelined, hipStream_t s);
 
 // ----- FFT field solve (pm_fft.hip) -------------------------------------------
 // Half spectrum layout: H[k * N + j], k in [0, N/2] (x frequency), j in [0, N)

@@ -28,8 +28,45 @@ __device__ __forceinline__ bool in_bounds(float x, float y, float hi) {
     return x >= 1.0f && x < hi && y >= 1.0f && y < hi;
 }
 
-__device__ __forceinline__ unsigned long long quantise(float p, float scale) {
-    return (unsigned long long)(long long)rintf(p * scale);
+// round(a * scale) as an unsigned 64-bit integer for a >= 0 and a * scale <
+// 2^64 (the mass shift keeps every contribution below 2^62).  v = rint(a s) is
+// split at 2^32: hi = floor(v 2^-32) and lo = v - hi 2^32 are exact floats (lo
+// keeps a subset of v's significand bits; v has a fraction only below 2^24,
+// where hi = 0), so two 32-bit conversions give the integer the generic
+// float -> int64 sequence gives, in 7 instructions instead of 12.
+__device__ __forceinline__ unsigned long long quantise_mag(float a, float scale) {
+    const float v = rintf(a * scale);
+    const float hi = floorf(v * 0x1p-32f);
+    const float lo = fmaf(hi, -0x1p32f, v);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// Fractional offset x - (int)x for x >= 1 (in-bounds positions): exact either
+// way, one instruction.
+__device__ __forceinline__ float frac(float x) { return __builtin_amdgcn_fractf(x); }
+
+// The four CIC contributions round(w * m * scale) of one particle, w the
+// reference weights (cuh:498-505).  Rounding is symmetric, so a negative mass
+// gives the negated contributions of |m|; the negation runs only in waves that
+// hold such a particle.
+struct Cic4 {
+    unsigned long long c00, c10, c01, c11;
+};
+__device__ __forceinline__ Cic4 cic_contrib(float fx, float fy, float pm, float scale) {
+    const float ax = 1.0f - fx, ay = 1.0f - fy, am = fabsf(pm);
+    Cic4 c{quantise_mag((ax * ay) * am, scale), quantise_mag((fx * ay) * am, scale),
+           quantise_mag((ax * fy) * am, scale), quantise_mag((fx * fy) * am, scale)};
+    if (__builtin_expect(__any(pm < 0.0f), 0) && pm < 0.0f) {
+        c.c00 = 0ull - c.c00;
+        c.c10 = 0ull - c.c10;
+        c.c01 = 0ull - c.c01;
+        c.c11 = 0ull - c.c11;
+    }
+    return c;
+}
+__device__ __forceinline__ unsigned long long ngp_contrib(float pm, float scale) {
+    const unsigned long long c = quantise_mag(fabsf(pm), scale);
+    return pm < 0.0f ? 0ull - c : c;
 }
 
 // CIC (MODE 1) or NGP (MODE 0) force at fractional offsets (fx, fy) in the
@@ -71,7 +108,7 @@ __device__ __forceinline__ void push_one(const float2* __restrict__ force, int n
                                          float& vx, float& vy) {
     if (!in_bounds(px, py, (float)(n - 1))) return;
     const int xi = (int)px, yi = (int)py;
-    const float2 a = gather<MODE>(force + (size_t)yi * n + xi, n, px - (float)xi, py - (float)yi);
+    const float2 a = gather<MODE>(force + (size_t)yi * n + xi, n, frac(px), frac(py));
     kick_drift<WRAP>(a, n, dt, px, py, vx, vy);
 }
 
@@ -88,15 +125,14 @@ __device__ __forceinline__ bool deposit_local(unsigned long long* win, int wx0, 
     if (!((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))) return false;
     unsigned long long* w = win + ly * W + lx;
     if (MODE == 0) {
-        atomicAdd(w, quantise(pm, scale));
+        atomicAdd(w, ngp_contrib(pm, scale));
         return true;
     }
-    const float fx = px - (float)xi, fy = py - (float)yi;
-    const float ax = 1.0f - fx, ay = 1.0f - fy;
-    atomicAdd(w, quantise((ax * ay) * pm, scale));
-    atomicAdd(w + 1, quantise((fx * ay) * pm, scale));
-    atomicAdd(w + W, quantise((ax * fy) * pm, scale));
-    atomicAdd(w + W + 1, quantise((fx * fy) * pm, scale));
+    const Cic4 c = cic_contrib(frac(px), frac(py), pm, scale);
+    atomicAdd(w, c.c00);
+    atomicAdd(w + 1, c.c10);
+    atomicAdd(w + W, c.c01);
+    atomicAdd(w + W + 1, c.c11);
     return true;
 }
 
@@ -106,15 +142,14 @@ __device__ __forceinline__ void deposit_global(unsigned long long* __restrict__ 
     const int xi = (int)px, yi = (int)py;
     unsigned long long* g = rho + (size_t)yi * n + xi;
     if (MODE == 0) {
-        atomicAdd(g, quantise(pm, scale));
+        atomicAdd(g, ngp_contrib(pm, scale));
         return;
     }
-    const float fx = px - (float)xi, fy = py - (float)yi;
-    const float ax = 1.0f - fx, ay = 1.0f - fy;
-    atomicAdd(g, quantise((ax * ay) * pm, scale));
-    atomicAdd(g + 1, quantise((fx * ay) * pm, scale));
-    atomicAdd(g + n, quantise((ax * fy) * pm, scale));
-    atomicAdd(g + n + 1, quantise((fx * fy) * pm, scale));
+    const Cic4 c = cic_contrib(frac(px), frac(py), pm, scale);
+    atomicAdd(g, c.c00);
+    atomicAdd(g + 1, c.c10);
+    atomicAdd(g + n, c.c01);
+    atomicAdd(g + n + 1, c.c11);
 }
 
 // Block c -> (bin b, slots [s0, s1)): chunk_start is the exclusive prefix of
@@ -202,9 +237,11 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit_bin(const float* __re
 // force window, or from the lattice when the cell lies outside it), write it
 // back, and deposit the new position into the next step's mass lattice.
 // COUNT: also histogram the new bins for the next re-sort.
+// PIPE: software-pipelined loads (below); 112 instead of 72 VGPRs, so two
+// blocks per CU instead of three.
 template <int TL> constexpr int fused_block() { return TL <= 5 ? 512 : 1024; }
 
-template <int TL, int MODE, int WRAP, bool COUNT>
+template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
 __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     const float2* __restrict__ force, float* __restrict__ x, float* __restrict__ y, float* __restrict__ vx,
     float* __restrict__ vy, const float* __restrict__ m, int n, float dt, const int64_t* __restrict__ start,
@@ -244,21 +281,51 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     const int lo_k = (int)(s0 - 4 * q0);             // first valid lane of group 0
     const int hi_k = (int)(s1 - 4 * (q1 - 1));       // valid lanes of the last group: [0, hi_k)
     unsigned int mig = 0;
+    // Each pass handles U groups per thread.  With PIPE the loads of the next
+    // pass are issued before this pass's gathers, pushes and deposits, so each
+    // wave keeps a pass of loads in flight while it computes: 4% faster at
+    // ~24 particles per cell, where the kernel is bound by HBM traffic, and
+    // 6-9% slower at ~240 per cell, where LDS atomic contention needs the
+    // third block per CU.
     constexpr int U = 2;
-    for (int j0 = threadIdx.x; j0 < nq; j0 += U * BLK) {
-        float px[U][4], py[U][4], ux[U][4], uy[U][4], pm[U][4];
-        bool ok[U][4];
+    v4f nx[U], ny[U], nvx[U], nvy[U], nm[U];
+    auto load_pass = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = j0 + u * BLK;
-            v4f a = {-1.f, -1.f, -1.f, -1.f}, b = a, c = {0.f, 0.f, 0.f, 0.f}, d = c, e = c;
+            nx[u] = v4f{-1.f, -1.f, -1.f, -1.f};
+            ny[u] = nx[u];
+            nvx[u] = v4f{0.f, 0.f, 0.f, 0.f};
+            nvy[u] = nvx[u];
+            nm[u] = nvx[u];
             if (j < nq) {
-                a = __builtin_nontemporal_load(X + j);
-                b = __builtin_nontemporal_load(Y + j);
-                c = __builtin_nontemporal_load(VX + j);
-                d = __builtin_nontemporal_load(VY + j);
-                e = __builtin_nontemporal_load(M + j);
+                nx[u] = __builtin_nontemporal_load(X + j);
+                ny[u] = __builtin_nontemporal_load(Y + j);
+                nvx[u] = __builtin_nontemporal_load(VX + j);
+                nvy[u] = __builtin_nontemporal_load(VY + j);
+                nm[u] = __builtin_nontemporal_load(M + j);
             }
+        }
+    };
+    if (PIPE) load_pass(threadIdx.x);
+    for (int j0 = threadIdx.x; j0 < nq; j0 += U * BLK) {
+        float px[U][4], py[U][4], ux[U][4], uy[U][4], pm[U][4];
+        bool ok[U][4];
+        v4f cx[U], cy[U], cvx[U], cvy[U], cm[U];
+        if (!PIPE) load_pass(j0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cx[u] = nx[u];
+            cy[u] = ny[u];
+            cvx[u] = nvx[u];
+            cvy[u] = nvy[u];
+            cm[u] = nm[u];
+        }
+        if (PIPE) load_pass(j0 + U * BLK);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * BLK;
+            const v4f a = cx[u], b = cy[u], c = cvx[u], d = cvy[u], e = cm[u];
             px[u][0] = a.x; px[u][1] = a.y; px[u][2] = a.z; px[u][3] = a.w;
             py[u][0] = b.x; py[u][1] = b.y; py[u][2] = b.z; py[u][3] = b.w;
             ux[u][0] = c.x; ux[u][1] = c.y; ux[u][2] = c.z; ux[u][3] = c.w;
@@ -275,7 +342,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
                 if (!ok[u][k] || !in_bounds(px[u][k], py[u][k], fhi)) continue;
                 const int xi = (int)px[u][k], yi = (int)py[u][k];
                 const int lx = xi - wx0, ly = yi - wy0;
-                const float fx = px[u][k] - (float)xi, fy = py[u][k] - (float)yi;
+                const float fx = frac(px[u][k]), fy = frac(py[u][k]);
                 float2 a;
                 if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
                     a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
@@ -426,7 +493,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
                 if (!ok[u][k] || !in_bounds(px[u][k], py[u][k], fhi)) continue;
                 const int xi = (int)px[u][k], yi = (int)py[u][k];
                 const int lx = xi - wx0, ly = yi - wy0;
-                const float fx = px[u][k] - (float)xi, fy = py[u][k] - (float)yi;
+                const float fx = frac(px[u][k]), fy = frac(py[u][k]);
                 float2 a;
                 if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
                     a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
@@ -602,41 +669,44 @@ hipError_t deposit_t(int n, int mode, const float* x, const float* y, const floa
     return hipGetLastError();
 }
 
+template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
+hipError_t push_deposit_k(int n, const float2* force, ParticleArrays p, BinState bs, int max_chunks, float dt,
+                          float scale, unsigned long long* r, unsigned long long* migrants, hipStream_t s) {
+    constexpr int W = TileGeom<TL>::kWin, BLK = fused_block<TL>();
+    const size_t lds = COUNT ? sizeof(int) * ((n >> TL) * (n >> TL) + 1) : 0;
+    if (COUNT && lds + 16 * W * W > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit<TL, MODE, WRAP, COUNT, PIPE>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_push_deposit<TL, MODE, WRAP, COUNT, PIPE>), dim3(max_chunks), dim3(BLK), lds, s, force,
+                       p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
+                       bs.count);
+    return hipGetLastError();
+}
+
 template <int TL, int MODE, int WRAP>
 hipError_t push_deposit_t(int n, const float2* force, ParticleArrays p, BinState bs, int max_chunks, float dt,
                           float scale, unsigned long long* r, unsigned long long* migrants, bool count_bins,
-                          hipStream_t s) {
-    constexpr int W = TileGeom<TL>::kWin, BLK = fused_block<TL>();
-    const int nb = (n >> TL) * (n >> TL) + 1;
-    if (count_bins) {
-        const size_t lds = sizeof(int) * nb;
-        if (lds + 16 * W * W > 65536) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit<TL, MODE, WRAP, true>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL((k_push_deposit<TL, MODE, WRAP, true>), dim3(max_chunks), dim3(BLK), lds, s, force, p.x,
-                           p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
-                           bs.count);
-    } else {
-        hipLaunchKernelGGL((k_push_deposit<TL, MODE, WRAP, false>), dim3(max_chunks), dim3(BLK), 0, s, force, p.x,
-                           p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
-                           bs.count);
-    }
-    return hipGetLastError();
+                          bool pipelined, hipStream_t s) {
+    if (count_bins)
+        return pipelined ? push_deposit_k<TL, MODE, WRAP, true, true>(n, force, p, bs, max_chunks, dt, scale, r, migrants, s)
+                         : push_deposit_k<TL, MODE, WRAP, true, false>(n, force, p, bs, max_chunks, dt, scale, r, migrants, s);
+    return pipelined ? push_deposit_k<TL, MODE, WRAP, false, true>(n, force, p, bs, max_chunks, dt, scale, r, migrants, s)
+                     : push_deposit_k<TL, MODE, WRAP, false, false>(n, force, p, bs, max_chunks, dt, scale, r, migrants, s);
 }
 
 template <int TL>
 hipError_t push_deposit_mode(int n, int mode, int wrap_mode, const float2* force, ParticleArrays p, BinState bs,
                              int max_chunks, float dt, float scale, unsigned long long* r,
-                             unsigned long long* migrants, bool count_bins, hipStream_t s) {
+                             unsigned long long* migrants, bool count_bins, bool pipelined, hipStream_t s) {
     if (mode == 0 && wrap_mode == 0)
-        return push_deposit_t<TL, 0, 0>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, s);
+        return push_deposit_t<TL, 0, 0>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, pipelined, s);
     if (mode == 0)
-        return push_deposit_t<TL, 0, 1>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, s);
+        return push_deposit_t<TL, 0, 1>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, pipelined, s);
     if (wrap_mode == 0)
-        return push_deposit_t<TL, 1, 0>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, s);
-    return push_deposit_t<TL, 1, 1>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, s);
+        return push_deposit_t<TL, 1, 0>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, pipelined, s);
+    return push_deposit_t<TL, 1, 1>(n, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, pipelined, s);
 }
 
 template <int MODE, int WRAP>
@@ -671,12 +741,13 @@ hipError_t deposit(int n, int mode, const float* x, const float* y, const float*
 
 hipError_t push_deposit(int n, int mode, int wrap_mode, const float2* force, ParticleArrays p, BinState bs,
                         int max_chunks, float dt, float scale, int64_t* rho_next, unsigned long long* migrants,
-                        bool count_bins, hipStream_t s) {
+                        bool count_bins, bool pipelined, hipStream_t s) {
     auto* r = reinterpret_cast<unsigned long long*>(rho_next);
     if (bs.tl == 5)
         return push_deposit_mode<5>(n, mode, wrap_mode, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins,
-                                    s);
-    return push_deposit_mode<6>(n, mode, wrap_mode, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins, s);
+                                    pipelined, s);
+    return push_deposit_mode<6>(n, mode, wrap_mode, force, p, bs, max_chunks, dt, scale, r, migrants, count_bins,
+                                pipelined, s);
 }
 
 int direct_blocks(int64_t np) { return (int)((np + 4 * kBlock - 1) / (4 * kBlock)); }

@@ -86,6 +86,61 @@ def test_deposit_bit_exact_c1(mode):
         assert np.array_equal(s.read_lattice(pm.LATTICE_MASS), want)
 
 
+@pytest.mark.parametrize("mode", [pm.NGP, pm.CIC])
+def test_deposit_signed_masses_and_rounding(mode):
+    """The fixed-point contributions round(w*m*2^shift) over their whole range,
+    bit-exact vs the oracle: mixed-sign masses (including +-0) with the
+    automatic shift, through the deposit and the fused step; exact .5 ties
+    (round half to even, both signs) at shift 1; contributions near 2^62 at
+    shift 60."""
+    n, npart = 256, 60000
+    rng = np.random.default_rng(11)
+    x = rng.uniform(8, 248, npart).astype(np.float32)
+    y = rng.uniform(8, 248, npart).astype(np.float32)
+    m = (rng.lognormal(0, 2.0, npart) * rng.choice([-1.0, 1.0], npart)).astype(np.float32)
+    m[:64] = 0.0
+    m[64:128] = -0.0
+    z = np.zeros_like(x)
+    with make_sim(n, npart, mode, dt=0.05) as s:
+        s.load(x, y, z, z, m)
+        s.stage(pm.STAGE_DEPOSIT)
+        shift = s.info().mass_shift
+        assert np.array_equal(s.read_lattice(pm.LATTICE_MASS), orc.deposit(n, mode, shift, x, y, m))
+        s.step(1)                                 # fused push + deposit of the new positions
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        rho = s.read_lattice(pm.LATTICE_MASS)
+    px, py, _, _ = orc.push32(n, mode, np.float32(0.05), 0, x, y, z, z, force)
+    assert np.array_equal(rho, orc.deposit(n, mode, shift, px, py, m))
+
+    # ties: at cell centres every CIC weight is 1/4; w*m*2 = k + 1/2 exactly
+    cells = np.arange(16, 240, 3, dtype=np.float32)
+    tx, ty = np.meshgrid(cells + 0.5, cells + 0.5)
+    tx, ty = tx.ravel(), ty.ravel()
+    tm = np.resize(np.float32([0.25, 0.75, 1.25, 1.75, -0.25, -0.75, -1.25, -1.75, 0.5, 3.0]), len(tx))
+    if mode == pm.CIC:
+        tm = tm * np.float32(4.0)
+    tz = np.zeros_like(tx)
+    with make_sim(n, len(tx), mode, mass_shift=1) as s:
+        s.load(tx, ty, tz, tz, tm)
+        s.stage(pm.STAGE_DEPOSIT)
+        got = s.read_lattice(pm.LATTICE_MASS)
+    want = orc.deposit(n, mode, 1, tx, ty, tm)
+    assert np.array_equal(got, want)
+    assert set(np.unique(want)) == {0, 1, 2, 4, 6, -2, -4}   # ties: 0.5 -> 0, 1.5 and 2.5 -> 2, 3.5 -> 4
+
+    # large: one particle per cell, contributions up to ~2^61.9
+    lx = np.arange(20, 220, 4, dtype=np.float32) + np.float32(0.0625)
+    ly = np.full_like(lx, 100.0625)
+    lm = np.linspace(0.5, 3.9, len(lx)).astype(np.float32) * np.resize(np.float32([1, -1]), len(lx))
+    lz = np.zeros_like(lx)
+    with make_sim(n, len(lx), mode, mass_shift=60) as s:
+        s.load(lx, ly, lz, lz, lm)
+        s.stage(pm.STAGE_DEPOSIT)
+        got = s.read_lattice(pm.LATTICE_MASS)
+    want = orc.deposit(n, mode, 60, lx, ly, lm)
+    assert np.array_equal(got, want) and np.abs(want).max() > 2 ** 61
+
+
 def test_deposit_hot_cell_and_empty():
     """All particles in one cell (maximum contention), and an empty load."""
     n, npart = 128, 200000
@@ -285,6 +340,29 @@ def test_fused_step_bit_exact(mode):
     n, npart = 512, 200000
     x, y, vx, vy, m = ics.exponential_disk(npart, 256, 256, 25, 200, seed=31, masses="lognormal")
     with make_sim(n, npart, mode, dt=0.05) as s:
+        s.load(x, y, vx, vy, m)
+        s.step(1)
+        force = s.read_lattice(pm.LATTICE_FORCE)
+        got = s.read()
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        shift = s.info().mass_shift
+    want = orc.push32(n, mode, np.float32(0.05), 0, x, y, vx, vy, force)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint32), w.view(np.uint32))
+    assert np.array_equal(rho, orc.deposit(n, mode, shift, want[0], want[1], m))
+
+
+@pytest.mark.parametrize("mode", [pm.NGP, pm.CIC])
+@pytest.mark.parametrize("interval", [8, 2])
+def test_fused_step_bit_exact_dense(mode, interval):
+    """Above 64 particles per cell the step kernel runs its non-pipelined
+    loop (interval 2: the variant that also counts bins for the next re-sort):
+    the same bit-exact bar at ~85 particles per cell."""
+    n, npart = 128, 1_400_000
+    x, y, vx, vy, m = ics.exponential_disk(npart, 64, 64, 8, 50, seed=32, masses="lognormal")
+    assert npart > 64 * n * n
+    with make_sim(n, npart, mode, dt=0.05) as s:
+        s.set_param("resort_interval", interval)
         s.load(x, y, vx, vy, m)
         s.step(1)
         force = s.read_lattice(pm.LATTICE_FORCE)

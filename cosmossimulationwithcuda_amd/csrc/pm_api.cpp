@@ -843,9 +843,12 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         PM_HIP(h, hipSetDevice(h->cfg.device));
         return sort_now(h, false);
     }
-    if (k == "fft_rp" || k == "fft_cb" || k == "fft_radix") {
+    if (k == "fft_rp" || k == "fft_cb" || k == "fft_radix" || k == "fft_it") {
         pmk::FftPlan p = h->plan;
-        (k == "fft_rp" ? p.rp : k == "fft_cb" ? p.cb : p.radix) = (int)value;
+        (k == "fft_rp" ? p.rp : k == "fft_cb" ? p.cb : k == "fft_radix" ? p.radix : p.it) = (int)value;
+        // the prefetch depth changes no result: fall back to 1 where the new
+        // batch or radix has no prefetching variant
+        if (k != "fft_it" && p.it != 1 && !pmk::plan_valid(h->cfg.n_lattice, p)) p.it = 1;
         if (!pmk::plan_valid(h->cfg.n_lattice, p)) return fail(h, PM_EINVAL, "FFT plan not available for this lattice");
         if ((p.radix == 16) != (h->cfg.precision == PM_FFT_BF16X3))
             return fail(h, PM_EINVAL, "fft_radix 16 is the PM_FFT_BF16X3 precision; 4 or 8 are FP32");
@@ -868,6 +871,7 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
     else if (k == "fft_rp") *value = h->plan.rp;
     else if (k == "fft_cb") *value = h->plan.cb;
     else if (k == "fft_radix") *value = h->plan.radix;
+    else if (k == "fft_it") *value = h->plan.it;
     else return fail(h, PM_EINVAL, "unknown parameter: " + k);
     return PM_OK;
 }
@@ -911,9 +915,10 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         float best_ms = 1e30f;
         for (int radix : {4, 8})
             for (int rp : {1, 2, 4})
-                for (int cb : {1, 2, 4, 8}) {
+                for (int cb : {1, 2, 4, 8})
+                for (int it : {1, 2, 4}) {
                     pmk::FftPlan p;
-                    p.rp = rp; p.cb = cb; p.radix = radix;
+                    p.rp = rp; p.cb = cb; p.radix = radix; p.it = it;
                     if (!pmk::plan_valid(n, p)) continue;
                     h->plan = p;
                     float ms = 0.f;
@@ -1155,6 +1160,14 @@ pm_status pm_checkpoint_restore(const void* src, int64_t bytes, int32_t device, 
     h->plan.rp = hd.fft_rp;
     h->plan.cb = hd.fft_cb;
     h->plan.radix = hd.fft_radix;
+    // the prefetch depth is not stored (it changes no result bit): take the
+    // default's where the stored plan allows it
+    h->plan.it = 1;
+    if (hd.fft_rp == 1 && hd.fft_cb == 1) {
+        pmk::FftPlan q = h->plan;
+        q.it = plan_for(c).it;
+        if (pmk::plan_valid(c.n_lattice, q)) h->plan = q;
+    }
     if (!pmk::plan_valid(c.n_lattice, h->plan)) h->plan = plan_for(c);
     h->resort_interval = hd.resort_interval > 0 ? hd.resort_interval : 8;
     h->resort_permille = hd.resort_permille;

@@ -34,333 +34,10 @@
 // Every pass reads its arrays from a FieldDesc; blockIdx.y selects the
 // instance, so one launch serves one simulation or a batch of simulations of
 // the same lattice size.
-#include "pm_internal.h"
+#include "pm_fft_core.h"
 
 namespace pmk {
 namespace {
-
-constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-// -i * a (forward quarter turn); +i * a is its negation
-__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
-
-template <bool INV>
-__device__ __forceinline__ void dft2(float2* v) {
-    const float2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-
-template <bool INV>
-__device__ __forceinline__ void dft4(float2* v) {
-    const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
-    const float2 r = mul_mi(t3);      // forward: X1 = t1 - i t3, X3 = t1 + i t3
-    v[0] = cadd(t0, t2);
-    v[2] = csub(t0, t2);
-    v[1] = INV ? csub(t1, r) : cadd(t1, r);
-    v[3] = INV ? cadd(t1, r) : csub(t1, r);
-}
-
-// 8-point DFT as a radix-2 split into two 4-point DFTs:
-// b_k = a_k + a_{k+4}, c_k = (a_k - a_{k+4}) W8^k, X_{2m} = DFT4(b)_m, X_{2m+1} = DFT4(c)_m.
-template <bool INV>
-__device__ __forceinline__ void dft8(float2* v) {
-    constexpr float h = 0.70710678118654752440f;
-    float2 b[4], c[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        b[k] = cadd(v[k], v[k + 4]);
-        c[k] = csub(v[k], v[k + 4]);
-    }
-    // W8^1 = (1 - i)/sqrt2 (forward), W8^2 = -i, W8^3 = (-1 - i)/sqrt2; inverse conjugates
-    if (!INV) {
-        c[1] = make_float2(h * (c[1].x + c[1].y), h * (c[1].y - c[1].x));
-        c[2] = mul_mi(c[2]);
-        c[3] = make_float2(h * (c[3].y - c[3].x), -h * (c[3].x + c[3].y));
-    } else {
-        c[1] = make_float2(h * (c[1].x - c[1].y), h * (c[1].y + c[1].x));
-        c[2] = make_float2(-c[2].y, c[2].x);
-        c[3] = make_float2(-h * (c[3].x + c[3].y), h * (c[3].x - c[3].y));
-    }
-    dft4<INV>(b);
-    dft4<INV>(c);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        v[2 * m] = b[m];
-        v[2 * m + 1] = c[m];
-    }
-}
-
-template <int R, bool INV>
-__device__ __forceinline__ void dft(float2* v) {
-    if constexpr (R == 2) dft2<INV>(v);
-    else if constexpr (R == 4) dft4<INV>(v);
-    else dft8<INV>(v);
-}
-
-// LDS layout: element i of a block's buffers is stored at pad(i) = i + i/32
-// (one spare slot per 32 elements), which spreads the strided Stockham writes
-// of the early stages over the LDS banks; strides that are multiples of 32
-// elements stay compile-time offsets.
-__device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
-constexpr int padded(int n) { return n + n / 32; }
-
-// Twiddles exp(-2 pi i m / N) for m in [0, N) from the quarter table
-// q[r] = exp(-2 pi i r / N), r < N/4, held in LDS: the quadrant factor
-// (-i)^(m / (N/4)) is a swap and sign change, so the values are exactly the
-// table's.
-template <int N, bool INV>
-__device__ __forceinline__ float2 twiddle(const float2* q, int m) {
-    constexpr int Q4 = N / 4, LQ = ilog2c(Q4);
-    const float2 w = q[m & (Q4 - 1)];
-    const unsigned qd = (unsigned)(m >> LQ) & 3u;
-    // quadrants 1 and 3 swap the components; signs: x flips for 2, 3; y for 1, 2
-    const bool sw = qd & 1u;
-    const unsigned sx = (qd >> 1) << 31, sy = ((qd ^ (qd >> 1)) & 1u) << 31;
-    const float x = sw ? w.y : w.x, y = sw ? w.x : w.y;
-    return make_float2(__uint_as_float(__float_as_uint(x) ^ sx),
-                       __uint_as_float(__float_as_uint(y) ^ sy ^ (INV ? 0x80000000u : 0u)));
-}
-
-// Row-group index of a row-pass block.  Workgroups are dispatched round-robin
-// over the 8 XCDs; the row passes touch the transposed half spectrum in short
-// segments (2*RP rows per kx), so consecutive row groups are given to
-// consecutive workgroups of the same XCD, whose L2 then serves the rest of
-// each 128-byte line to the neighbouring groups.
-__device__ __forceinline__ int row_group() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
-}
-
-// Loads the quarter twiddle table into LDS (callers synchronise after).
-template <int N>
-__device__ __forceinline__ void load_twiddles(float2* q, const float2* __restrict__ tw) {
-    for (int e = threadIdx.x; e < N / 4; e += kBlock) q[e] = tw[e];
-}
-
-// One Stockham stage of radix R with sub-transform length NS over NFFT
-// in-place transforms of length N at buf + f*N (padded).  Reads, barrier,
-// writes, barrier.
-template <int N, int NFFT, int R, int NS, bool INV>
-__device__ __forceinline__ void stage(float2* buf, const float2* tw) {
-    constexpr int Q = N / R, NB = NFFT * Q, PER = (NB + kBlock - 1) / kBlock;
-    constexpr int STEP = N / (R * NS);
-    const int t = threadIdx.x;
-    float2 v[PER][R];
-    int ob[PER];
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-        const int bb = t + p * kBlock;
-        if (NB % kBlock == 0 || bb < NB) {
-            const int f = bb / Q, b = bb - f * Q;
-            if constexpr (Q % 32 == 0) {
-                const float2* src = buf + pad(f * N + b);
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[p][r] = src[r * padded(Q)];
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[p][r] = buf[pad(f * N + b + r * Q)];
-            }
-            const int k = b & (NS - 1);
-            if (NS > 1) {
-                const int e = k * STEP;
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], twiddle<N, INV>(tw, r * e));
-            }
-            dft<R, INV>(v[p]);
-            ob[p] = f * N + (b - k) * R + k;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-        const int bb = t + p * kBlock;
-        if (NB % kBlock == 0 || bb < NB) {
-            if constexpr (NS % 32 == 0) {
-                float2* dst = buf + pad(ob[p]);
-#pragma unroll
-                for (int r = 0; r < R; ++r) dst[r * padded(NS)] = v[p][r];
-            } else if constexpr (32 % (NS * R) == 0) {   // the R outputs share one 32-element row
-                float2* dst = buf + pad(ob[p]);
-#pragma unroll
-                for (int r = 0; r < R; ++r) dst[r * NS] = v[p][r];
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; ++r) buf[pad(ob[p] + r * NS)] = v[p][r];
-            }
-        }
-    }
-    __syncthreads();
-}
-
-template <int N, int NFFT, int R, int NS, bool INV>
-__device__ __forceinline__ void stages_from(float2* buf, const float2* tw) {
-    if constexpr (NS < N) {
-        stage<N, NFFT, R, NS, INV>(buf, tw);
-        stages_from<N, NFFT, R, NS * R, INV>(buf, tw);
-    }
-}
-
-// NFFT in-place transforms of length N at buf + f*N (padded), radix RADIX
-// (4 or 8) after a leading stage for the leftover factor; tw is the LDS quarter
-// table.  Caller synchronises before; ends with a barrier.
-// ---- split-BF16 matrix-core radix-16 stages (precision variant) --------------
-// A radix-16 stage over a group of 16 butterflies is one complex 16x16 matrix
-// product X = F * x (F[m][r] = exp(-+2 pi i m r / 16)), done as the real
-// product [Xr; Xi] = [[Fr, -Fi], [Fi, Fr]] [xr; xi] on the matrix cores with
-// v_mfma_f32_16x16x16_bf16: every FP32 operand is split into bf16 hi + lo
-// parts and each real product is formed as hi*hi + hi*lo + lo*hi with FP32
-// accumulation (the lo*lo term, ~2^-16 relative, is dropped).  Operand
-// layout (16x16x16): lane l holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16]
-// for j < 4, and receives D[4(l/16)+i][l%16] for i < 4.
-typedef short v4bf __attribute__((ext_vector_type(4)));
-typedef float v4acc __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ short bf16_rne(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
-__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
-    hi = bf16_rne(x);
-    lo = bf16_rne(x - __uint_as_float((unsigned)(unsigned short)hi << 16));
-}
-
-// cos(2 pi j / 16), j < 16
-__constant__ float kCos16[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
-                                 0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
-                                 -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
-                                 0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
-
-// DFT-16 operands: [h][kh] = output half (re, im) x input half (re, im).
-struct Dft16Ops {
-    v4bf hi[2][2], lo[2][2];
-};
-
-template <bool INV>
-__device__ __forceinline__ void dft16_ops(Dft16Ops& A) {
-    const int l = threadIdx.x & 63, m = l & 15, g = l >> 4;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int idx = (m * (4 * g + j)) & 15;
-        const float c = kCos16[idx], sn = kCos16[(idx + 12) & 15];   // sin(2 pi idx/16)
-        const float fr = c, fi = INV ? sn : -sn;
-        const float v[2][2] = {{fr, -fi}, {fi, fr}};
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int kh = 0; kh < 2; ++kh) {
-                short a, b;
-                split_bf16(v[h][kh], a, b);
-                A.hi[h][kh][j] = a;
-                A.lo[h][kh][j] = b;
-            }
-    }
-}
-
-// One radix-16 Stockham stage (same indexing as stage<>), butterflies taken
-// 16 at a time per wavefront.  Results are held in registers until every
-// wavefront has read its inputs (in place, like stage<>).
-template <int N, int NFFT, int NS, bool INV>
-__device__ __forceinline__ void stage16(float2* buf, const float2* tw, const Dft16Ops& A) {
-    constexpr int R = 16, Q = N / R, NB = NFFT * Q, G = NB / 16, NW = kBlock / 64, GPW = (G + NW - 1) / NW;
-    constexpr int STEP = N / (R * NS);
-    static_assert(NB % 16 == 0, "radix-16 matrix stage needs a multiple of 16 butterflies");
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, c = l & 15, g = l >> 4;
-    v4acc dre[GPW], dim[GPW];
-    int ob[GPW];
-#pragma unroll
-    for (int q = 0; q < GPW; ++q) {
-        const int grp = w + q * NW;
-        if (G % NW == 0 || grp < G) {              // uniform per wavefront
-            const int bb = grp * 16 + c;
-            const int f = bb / Q, b = bb - f * Q, k = b & (NS - 1);
-            v4bf bhi[2], blo[2];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = 4 * g + j;
-                float2 x = buf[pad(f * N + b + r * Q)];
-                if (NS > 1) x = cmul(x, twiddle<N, INV>(tw, r * k * STEP));
-                short h0, l0, h1, l1;
-                split_bf16(x.x, h0, l0);
-                split_bf16(x.y, h1, l1);
-                bhi[0][j] = h0; blo[0][j] = l0;
-                bhi[1][j] = h1; blo[1][j] = l1;
-            }
-            v4acc acc[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                acc[h] = v4acc{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kh = 0; kh < 2; ++kh) {
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.lo[h][kh], bhi[kh], acc[h], 0, 0, 0);
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.hi[h][kh], blo[kh], acc[h], 0, 0, 0);
-                }
-#pragma unroll
-                for (int kh = 0; kh < 2; ++kh)
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.hi[h][kh], bhi[kh], acc[h], 0, 0, 0);
-            }
-            dre[q] = acc[0];
-            dim[q] = acc[1];
-            ob[q] = f * N + (b - k) * R + k;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < GPW; ++q) {
-        const int grp = w + q * NW;
-        if (G % NW == 0 || grp < G) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) buf[pad(ob[q] + (4 * g + i) * NS)] = make_float2(dre[q][i], dim[q][i]);
-        }
-    }
-    __syncthreads();
-}
-
-template <int N, int NFFT, int NS, bool INV>
-__device__ __forceinline__ void stages16_from(float2* buf, const float2* tw, const Dft16Ops& A) {
-    if constexpr (NS < N) {
-        stage16<N, NFFT, NS, INV>(buf, tw, A);
-        stages16_from<N, NFFT, NS * 16, INV>(buf, tw, A);
-    }
-}
-
-// NFFT transforms (as lds_fft) with radix-16 matrix-core stages after one
-// leading FP32 stage of radix 2, 4 or 8 for the leftover factor.
-template <int N, int NFFT, bool INV>
-__device__ __forceinline__ void lds_fft_mfma(float2* buf, const float2* tw) {
-    constexpr int L = ilog2c(N), REM = L % 4;
-    Dft16Ops A;
-    dft16_ops<INV>(A);
-    if constexpr (REM == 0) {
-        stages16_from<N, NFFT, 1, INV>(buf, tw, A);
-    } else {
-        constexpr int R0 = 1 << REM;
-        stage<N, NFFT, R0, 1, INV>(buf, tw);
-        stages16_from<N, NFFT, R0, INV>(buf, tw, A);
-    }
-}
-
-// RADIX 4 or 8: FP32 Stockham stages; RADIX 16: the split-BF16 matrix-core
-// variant above.
-template <int N, int NFFT, int RADIX, bool INV>
-__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
-    constexpr int LR = ilog2c(RADIX), L = ilog2c(N), REM = L % LR;
-    if constexpr (RADIX == 16) {
-        lds_fft_mfma<N, NFFT, INV>(buf, tw);
-    } else if constexpr (REM == 0) {
-        stages_from<N, NFFT, RADIX, 1, INV>(buf, tw);
-    } else {
-        constexpr int R0 = 1 << REM;
-        stage<N, NFFT, R0, 1, INV>(buf, tw);
-        stages_from<N, NFFT, RADIX, R0, INV>(buf, tw);
-    }
-}
 
 // ------------------------------------------------------------------ row R2C --
 template <int N, int RP, int RADIX>
@@ -368,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
                                                      const float2* __restrict__ tw) {
     __shared__ float2 buf[padded(RP * N)];
     __shared__ float2 twl[N / 4];
-    const FieldDesc d = descs[blockIdx.y];
+    const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
     load_twiddles<N>(twl, tw);
@@ -377,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
         buf[pad(e)] = make_float2((float)d.rho[ra] * d.inv_scale, (float)d.rho[ra + N] * d.inv_scale);
     }
-    __syncthreads();
+    lds_barrier();
     lds_fft<N, RP, RADIX, false>(buf, twl);
     constexpr int NK = N / 2 + 1;
     for (int e = t; e < NK * RP; e += kBlock) {
@@ -387,8 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
         // Xa = (Z + conj(Zc))/2,  Xb = (Z - conj(Zc))/(2i)
         const float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
         const float2 xb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
-        float4* dst = reinterpret_cast<float4*>(d.H + (size_t)k * N + j0 + 2 * f);
-        *dst = make_float4(xa.x, xa.y, xb.x, xb.y);
+        *(gptr<cplx2>)(d.H + (size_t)k * N + j0 + 2 * f) = cplx2{xa.x, xa.y, xb.x, xb.y};
     }
 }
 
@@ -404,16 +80,16 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
                                                       const float2* __restrict__ tw) {
     __shared__ float2 buf[padded((FORCE ? 2 : 1) * CB * N)];
     __shared__ float2 twl[N / 4];
-    const FieldDesc d = descs[blockIdx.y];
+    const GField d = gfield(descs[blockIdx.y]);
     constexpr int NY = N / 2;                 // the Nyquist column's kx
     const int t = threadIdx.x;
     const int k0 = blockIdx.x * CB;
     load_twiddles<N>(twl, tw);
     for (int e = t; e < CB * N; e += kBlock) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
-        buf[pad(e)] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : d.H[(size_t)k * N + ky];
+        buf[pad(e)] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : f2(d.H[(size_t)k * N + ky]);
     }
-    __syncthreads();
+    lds_barrier();
     lds_fft<N, CB, RADIX, false>(buf, twl);
     for (int e = (k0 == 0 ? N : 0) + t; e < CB * N; e += kBlock) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
@@ -446,22 +122,22 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     lds_fft<N, (FORCE ? 2 : 1) * CB, RADIX, true>(buf, twl);
     for (int e = t; e < CB * N; e += kBlock) {
         const int f = e / N, j = e - f * N, k = k0 + f;
         const float2 w = buf[pad(e)];
         const float2 v = FORCE ? buf[pad(CB * N + e)] : make_float2(0.f, 0.f);
         if (k == 0) {
-            d.H[j] = make_float2(w.x, 0.f);
-            d.H[(size_t)NY * N + j] = make_float2(w.y, 0.f);
+            d.H[j] = cplx{w.x, 0.f};
+            d.H[(size_t)NY * N + j] = cplx{w.y, 0.f};
             if (FORCE) {
-                d.Hfy[j] = make_float2(v.x, 0.f);
-                d.Hfy[(size_t)NY * N + j] = make_float2(v.y, 0.f);
+                d.Hfy[j] = cplx{v.x, 0.f};
+                d.Hfy[(size_t)NY * N + j] = cplx{v.y, 0.f};
             }
         } else {
-            d.H[(size_t)k * N + j] = w;
-            if (FORCE) d.Hfy[(size_t)k * N + j] = v;
+            d.H[(size_t)k * N + j] = nv(w);
+            if (FORCE) d.Hfy[(size_t)k * N + j] = nv(v);
         }
     }
 }
@@ -477,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
     constexpr bool MERGE = FORCE && N <= 1024 && padded(3 * RP * N) * 8 + 2 * N <= 160 * 1024;
     __shared__ float2 buf[padded((MERGE ? 3 : (FORCE ? 2 : 1)) * RP * N)];
     __shared__ float2 twl[N / 4];
-    const FieldDesc d = descs[blockIdx.y];
+    const GField d = gfield(descs[blockIdx.y]);
     constexpr int NK = N / 2 + 1;
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
@@ -486,8 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
         for (int e = t; e < NK * RP; e += kBlock) {
             const int k = e / RP, f = e - k * RP;
             const size_t src = (size_t)k * N + j0 + 2 * f;
-            const float4 v = *reinterpret_cast<const float4*>(d.H + src);
-            const float4 w = *reinterpret_cast<const float4*>(d.Hfy + src);
+            const cplx2 v = *(gptr<const cplx2>)(d.H + src);
+            const cplx2 w = *(gptr<const cplx2>)(d.Hfy + src);
             const bool edge = k == 0 || k == N / 2;
             float2 xa = make_float2(v.x, v.y), xb = make_float2(v.z, v.w);
             if (edge) { xa.y = 0.f; xb.y = 0.f; }
@@ -505,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
                 if (!edge) buf[pad(row * N + N - k)] = make_float2(fx.x + fy.y, fy.x - fx.y);
             }
         }
-        __syncthreads();
+        lds_barrier();
         lds_fft<N, 3 * RP, RADIX, true>(buf, twl);
         for (int e = t; e < RP * N; e += kBlock) {
             const int f = e / N, i = e - f * N;
@@ -516,20 +192,20 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
         }
         for (int e = t; e < 2 * RP * N; e += kBlock) {
             const int q = e / N, i = e - q * N;
-            d.force[(size_t)(j0 + q) * N + i] = buf[pad(RP * N + e)];
+            d.force[(size_t)(j0 + q) * N + i] = nv(buf[pad(RP * N + e)]);
         }
         return;
     }
     // potential: rows (j0+2f, j0+2f+1) packed as one complex row
     for (int e = t; e < NK * RP; e += kBlock) {
         const int k = e / RP, f = e - k * RP;
-        const float4 v = *reinterpret_cast<const float4*>(d.H + (size_t)k * N + j0 + 2 * f);
+        const cplx2 v = *(gptr<const cplx2>)(d.H + (size_t)k * N + j0 + 2 * f);
         float2 xa = make_float2(v.x, v.y), xb = make_float2(v.z, v.w);
         if (k == 0 || k == N / 2) { xa.y = 0.f; xb.y = 0.f; }
         buf[pad(f * N + k)] = make_float2(xa.x - xb.y, xa.y + xb.x);
         if (k > 0 && k < N / 2) buf[pad(f * N + N - k)] = make_float2(xa.x + xb.y, xb.x - xa.y);
     }
-    __syncthreads();
+    lds_barrier();
     lds_fft<N, RP, RADIX, true>(buf, twl);
     for (int e = t; e < RP * N; e += kBlock) {
         const int f = e / N, i = e - f * N;
@@ -539,25 +215,25 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
         d.phi[ra + N] = z.y;
     }
     if (!FORCE) return;
-    __syncthreads();
+    lds_barrier();
     // force: one row per complex FFT, Z = Fx + i Fy
     constexpr int R2 = 2 * RP;
     for (int e = t; e < NK * R2; e += kBlock) {
         const int k = e / R2, q = e - k * R2;
         const size_t src = (size_t)k * N + j0 + q;
-        const float2 p = d.H[src];
-        float2 fy = d.Hfy[src];
+        const float2 p = f2(d.H[src]);
+        float2 fy = f2(d.Hfy[src]);
         const float dv = d.dk[k];
         const float2 fx = make_float2(-p.y * dv, p.x * dv);     // i * d(kx) * Phi-hat
         if (k == 0 || k == N / 2) fy.y = 0.f;
         buf[pad(q * N + k)] = make_float2(fx.x - fy.y, fx.y + fy.x);
         if (k > 0 && k < N / 2) buf[pad(q * N + N - k)] = make_float2(fx.x + fy.y, fy.x - fx.y);
     }
-    __syncthreads();
+    lds_barrier();
     lds_fft<N, R2, RADIX, true>(buf, twl);
     for (int e = t; e < R2 * N; e += kBlock) {
         const int q = e / N, i = e - q * N;
-        d.force[(size_t)(j0 + q) * N + i] = buf[pad(e)];
+        d.force[(size_t)(j0 + q) * N + i] = nv(buf[pad(e)]);
     }
 }
 
@@ -705,6 +381,7 @@ hipError_t stencil_n(const FieldDesc* d, int count, hipStream_t s) {
     }
 
 bool plan_valid(int n, FftPlan p) {
+    if (p.it != 1) return p.rp == 1 && p.cb == 1 && pipe_plan_compiled(n, p.radix, p.it);
     if (p.radix == 16) return p.rp == 1 && p.cb == 1 && n >= 256 && n <= 8192;
     if (p.radix != 4 && p.radix != 8) return false;
     if (p.rp != 1 && p.rp != 2 && p.rp != 4) return false;
@@ -714,26 +391,32 @@ bool plan_valid(int n, FftPlan p) {
 }
 
 // Measured on MI355X over n = 256..4096: one row pair / one column per block
-// (the most blocks in flight) is fastest at every size; radix 8 from 2048 up.
+// (the most blocks in flight) is fastest at every size; radix 8 from 2048 up;
+// at 2048 two row pairs / columns per block with the next one prefetched
+// take the field solve from 107 to 102 us (slower at 1024 and 4096).
 FftPlan default_plan(int n) {
     FftPlan p;
     p.radix = n >= 2048 ? 8 : 4;
     p.rp = 1;
     p.cb = 1;
+    p.it = n == 2048 ? 2 : 1;
     return p;
 }
 
 hipError_t row_r2c(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s) {
+    if (p.it != 1) return row_r2c_pipe(n, d, count, tw, p, s);
     PM_DISPATCH_N(n, (r2c_n<NN>(d, count, tw, p, s)));
 }
 
 hipError_t col_conv(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
                     hipStream_t s) {
+    if (p.it != 1) return col_conv_pipe(n, d, count, tw, p, with_force, s);
     PM_DISPATCH_N(n, (col_n<NN>(d, count, tw, p, with_force, s)));
 }
 
 hipError_t row_c2r(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
                    hipStream_t s) {
+    if (p.it != 1) return row_c2r_pipe(n, d, count, tw, p, with_force, s);
     PM_DISPATCH_N(n, (c2r_n<NN>(d, count, tw, p, with_force, s)));
 }
 

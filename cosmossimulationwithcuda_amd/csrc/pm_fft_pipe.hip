@@ -1,0 +1,277 @@
+// The three FFT passes with IT row pairs (row passes) or IT columns (column
+// pass) per block, processed in sequence: the global inputs of the next row
+// pair / column are loaded into registers before the current one's LDS
+// transform, so its HBM reads overlap the transform instead of every block
+// loading, transforming and storing in lockstep.  Same arithmetic per
+// transform as the one-per-block passes of pm_fft.hip (identical results);
+// the C2R pass stages the potential row pair and both force rows of a pair in
+// one batch (three transforms), which frees the prefetch registers before the
+// transform starts.  Plan field `it` (rp = cb = 1); compiled for the default
+// radix at N = 1024 (radix 4), 2048 and 4096 (radix 8), IT in {2, 4}.
+#include "pm_fft_core.h"
+
+namespace pmk {
+namespace {
+
+// ------------------------------------------------------------------ row R2C --
+template <int N, int RADIX, int IT>
+__global__ __launch_bounds__(kBlock) void k_row_r2c_it(const FieldDesc* __restrict__ descs,
+                                                        const float2* __restrict__ tw) {
+    __shared__ float2 buf[padded(N)];
+    __shared__ float2 twl[N / 4];
+    constexpr int PER = N / kBlock, NK = N / 2 + 1;
+    const GField d = gfield(descs[blockIdx.y]);
+    const int t = threadIdx.x;
+    const int g0 = row_group() * IT;            // row pairs g0 .. g0 + IT - 1
+    int64_t ra[PER], rb[PER];
+    auto fetch = [&](int g) {
+        const gptr<const int64_t> r = d.rho + (size_t)(2 * g) * N + t;
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            ra[p] = r[p * kBlock];
+            rb[p] = r[N + p * kBlock];
+        }
+    };
+    load_twiddles<N>(twl, tw);
+    fetch(g0);
+    for (int it = 0; it < IT; ++it) {
+        const int j0 = 2 * (g0 + it);
+#pragma unroll
+        for (int p = 0; p < PER; ++p)
+            buf[pad(t + p * kBlock)] = make_float2((float)ra[p] * d.inv_scale, (float)rb[p] * d.inv_scale);
+        lds_barrier();
+        if (it + 1 < IT) fetch(g0 + it + 1);    // in flight during the transform
+        lds_fft<N, 1, RADIX, false>(buf, twl);
+        for (int k = t; k < NK; k += kBlock) {
+            const float2 z = buf[pad(k)];
+            const float2 zc = buf[pad((N - k) & (N - 1))];
+            // Xa = (Z + conj(Zc))/2,  Xb = (Z - conj(Zc))/(2i)
+            const float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+            const float2 xb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+            *(gptr<cplx2>)(d.H + (size_t)k * N + j0) = cplx2{xa.x, xa.y, xb.x, xb.y};
+        }
+        lds_barrier();                         // buf is rewritten by the next row pair
+    }
+}
+
+// --------------------------------------------------------------- column pass --
+// Columns kx = blockIdx.x * IT + it; kx = 0 carries the packed kx = 0 / N/2
+// pair exactly as the one-column pass does.
+template <int N, int RADIX, bool FORCE, int IT>
+__global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restrict__ descs,
+                                                         const float2* __restrict__ tw) {
+    __shared__ float2 buf[padded((FORCE ? 2 : 1) * N)];
+    __shared__ float2 twl[N / 4];
+    constexpr int NY = N / 2, PER = N / kBlock;
+    const GField d = gfield(descs[blockIdx.y]);
+    const int t = threadIdx.x;
+    const int kb = blockIdx.x * IT;
+    float2 h[PER];
+    float tv[PER];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int ky = t + p * kBlock;
+            h[p] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : f2(d.H[(size_t)k * N + ky]);
+            tv[p] = d.T[(size_t)k * N + ky];
+        }
+    };
+    load_twiddles<N>(twl, tw);
+    fetch(kb);
+    for (int it = 0; it < IT; ++it) {
+        const int k = kb + it;
+        float tc[PER];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            buf[pad(t + p * kBlock)] = h[p];
+            tc[p] = tv[p];
+        }
+        lds_barrier();
+        if (it + 1 < IT) fetch(k + 1);           // in flight during the transforms
+        lds_fft<N, 1, RADIX, false>(buf, twl);
+        if (k != 0) {
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                const int e = t + p * kBlock;
+                const float2 x = buf[pad(e)];
+                const float2 y = make_float2(x.x * tc[p], x.y * tc[p]);
+                buf[pad(e)] = y;
+                if (FORCE) {
+                    const float dv = d.dk[e];
+                    buf[pad(N + e)] = make_float2(-y.y * dv, y.x * dv);   // i * d * y
+                }
+            }
+        } else {                                 // packed kx = 0 / N/2 slot, by conjugate pairs
+            for (int a = t; a <= N / 2; a += kBlock) {
+                const int b = (N - a) & (N - 1);
+                const float2 za = buf[pad(a)], zb = buf[pad(b)];
+                const float2 A = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+                const float2 B = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
+                const float t0a = d.T[a], t0b = d.T[b];
+                const float tna = d.T[(size_t)NY * N + a], tnb = d.T[(size_t)NY * N + b];
+                // W(a) = A T0 + i B TN;  W(b) = conj(A) T0 + i conj(B) TN
+                const float2 wa = make_float2(A.x * t0a - B.y * tna, A.y * t0a + B.x * tna);
+                const float2 wb = make_float2(A.x * t0b + B.y * tnb, -A.y * t0b + B.x * tnb);
+                buf[pad(a)] = wa;
+                buf[pad(b)] = wb;
+                if (FORCE) {
+                    const float da = d.dk[a], db = d.dk[b];
+                    buf[pad(N + a)] = make_float2(-wa.y * da, wa.x * da);
+                    buf[pad(N + b)] = make_float2(-wb.y * db, wb.x * db);
+                }
+            }
+        }
+        lds_barrier();
+        lds_fft<N, FORCE ? 2 : 1, RADIX, true>(buf, twl);
+        for (int j = t; j < N; j += kBlock) {
+            const float2 w = buf[pad(j)];
+            const float2 v = FORCE ? buf[pad(N + j)] : make_float2(0.f, 0.f);
+            if (k == 0) {
+                d.H[j] = cplx{w.x, 0.f};
+                d.H[(size_t)NY * N + j] = cplx{w.y, 0.f};
+                if (FORCE) {
+                    d.Hfy[j] = cplx{v.x, 0.f};
+                    d.Hfy[(size_t)NY * N + j] = cplx{v.y, 0.f};
+                }
+            } else {
+                d.H[(size_t)k * N + j] = nv(w);
+                if (FORCE) d.Hfy[(size_t)k * N + j] = nv(v);
+            }
+        }
+        lds_barrier();                         // buf is rewritten by the next column
+    }
+}
+
+// ------------------------------------------------------------------ row C2R --
+// Per row pair: the potential rows (two per complex transform) and, with
+// FORCE, the force rows (Z = Fx + i Fy per row, Fx-hat = i d(kx) Phi-hat) as
+// one batch of 1 or 3 transforms.
+template <int N, int RADIX, bool FORCE, int IT>
+__global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restrict__ descs,
+                                                        const float2* __restrict__ tw) {
+    constexpr int ROWS = FORCE ? 3 : 1, NK = N / 2 + 1, PK = (NK + kBlock - 1) / kBlock;
+    __shared__ float2 buf[padded(ROWS * N)];
+    __shared__ float2 twl[N / 4];
+    const GField d = gfield(descs[blockIdx.y]);
+    const int t = threadIdx.x;
+    const int g0 = row_group() * IT;
+    cplx2 hv[PK], hw[PK];
+    auto fetch = [&](int g) {
+#pragma unroll
+        for (int q = 0; q < PK; ++q) {
+            const int k = t + q * kBlock;
+            if (k < NK) {
+                const size_t src = (size_t)k * N + 2 * g;
+                hv[q] = *(gptr<const cplx2>)(d.H + src);
+                if (FORCE) hw[q] = *(gptr<const cplx2>)(d.Hfy + src);
+            }
+        }
+    };
+    load_twiddles<N>(twl, tw);
+    fetch(g0);
+    for (int it = 0; it < IT; ++it) {
+        const int j0 = 2 * (g0 + it);
+#pragma unroll
+        for (int q = 0; q < PK; ++q) {
+            const int k = t + q * kBlock;
+            if (k >= NK) continue;
+            const cplx2 v = hv[q];
+            const bool edge = k == 0 || k == N / 2;
+            float2 xa = make_float2(v.x, v.y), xb = make_float2(v.z, v.w);
+            if (edge) { xa.y = 0.f; xb.y = 0.f; }
+            buf[pad(k)] = make_float2(xa.x - xb.y, xa.y + xb.x);
+            if (!edge) buf[pad(N - k)] = make_float2(xa.x + xb.y, xb.x - xa.y);
+            if (FORCE) {
+                const cplx2 w = hw[q];
+                const float dv = d.dk[k];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const float2 p = r ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+                    float2 fy = r ? make_float2(w.z, w.w) : make_float2(w.x, w.y);
+                    const float2 fx = make_float2(-p.y * dv, p.x * dv);     // i * d(kx) * Phi-hat
+                    if (edge) fy.y = 0.f;
+                    const int row = 1 + r;
+                    buf[pad(row * N + k)] = make_float2(fx.x - fy.y, fx.y + fy.x);
+                    if (!edge) buf[pad(row * N + N - k)] = make_float2(fx.x + fy.y, fy.x - fx.y);
+                }
+            }
+        }
+        lds_barrier();
+        if (it + 1 < IT) fetch(g0 + it + 1);    // in flight during the transforms
+        lds_fft<N, ROWS, RADIX, true>(buf, twl);
+        for (int i = t; i < N; i += kBlock) {
+            const size_t ra = (size_t)j0 * N + i;
+            const float2 z = buf[pad(i)];
+            d.phi[ra] = z.x;
+            d.phi[ra + N] = z.y;
+        }
+        if (FORCE)
+            for (int e = t; e < 2 * N; e += kBlock) {
+                const int q = e / N, i = e - q * N;
+                d.force[(size_t)(j0 + q) * N + i] = nv(buf[pad(N + e)]);
+            }
+        lds_barrier();                         // buf is rewritten by the next row pair
+    }
+}
+
+template <int N, int RADIX, int IT>
+hipError_t launch_r2c(const FieldDesc* d, int count, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_row_r2c_it<N, RADIX, IT>), dim3(N / 2 / IT, count), dim3(kBlock), 0, s, d, tw);
+    return hipGetLastError();
+}
+template <int N, int RADIX, int IT>
+hipError_t launch_col(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
+    const dim3 grid(N / 2 / IT, count);
+    if (force) hipLaunchKernelGGL((k_col_conv_it<N, RADIX, true, IT>), grid, dim3(kBlock), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_col_conv_it<N, RADIX, false, IT>), grid, dim3(kBlock), 0, s, d, tw);
+    return hipGetLastError();
+}
+template <int N, int RADIX, int IT>
+hipError_t launch_c2r(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
+    const dim3 grid(N / 2 / IT, count);
+    if (force) hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, true, IT>), grid, dim3(kBlock), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, false, IT>), grid, dim3(kBlock), 0, s, d, tw);
+    return hipGetLastError();
+}
+
+// pass 0 = R2C, 1 = column, 2 = C2R
+template <int N, int RADIX, int IT>
+hipError_t launch_pass(int pass, const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
+    static_assert(padded(3 * N) * 8 + 2 * N <= 160 * 1024, "C2R batch must fit LDS");
+    if (pass == 0) return launch_r2c<N, RADIX, IT>(d, count, tw, s);
+    if (pass == 1) return launch_col<N, RADIX, IT>(d, count, tw, force, s);
+    return launch_c2r<N, RADIX, IT>(d, count, tw, force, s);
+}
+
+hipError_t pipe_pass(int pass, int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool force,
+                     hipStream_t s) {
+#define PM_PIPE(NV, RV, ITV) \
+    if (n == NV && p.radix == RV && p.it == ITV) return launch_pass<NV, RV, ITV>(pass, d, count, tw, force, s);
+    PM_PIPE(1024, 4, 2) PM_PIPE(1024, 4, 4) PM_PIPE(2048, 8, 2) PM_PIPE(2048, 8, 4) PM_PIPE(4096, 8, 2)
+    PM_PIPE(4096, 8, 4)
+#undef PM_PIPE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+bool pipe_plan_compiled(int n, int radix, int it) {
+    if (it != 2 && it != 4) return false;
+    return (n == 1024 && radix == 4) || ((n == 2048 || n == 4096) && radix == 8);
+}
+
+hipError_t row_r2c_pipe(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s) {
+    return pipe_pass(0, n, d, count, tw, p, false, s);
+}
+
+hipError_t col_conv_pipe(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
+                         hipStream_t s) {
+    return pipe_pass(1, n, d, count, tw, p, with_force, s);
+}
+
+hipError_t row_c2r_pipe(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
+                        hipStream_t s) {
+    return pipe_pass(2, n, d, count, tw, p, with_force, s);
+}
+
+}  // namespace pmk

@@ -114,11 +114,19 @@ struct FieldDesc {
 };
 // Tile batch and radix of the three passes: rp row pairs per row-pass block,
 // cb columns per column-pass block, radix 4 or 8 (FP32 Stockham stages) or 16
-// (split-BF16 matrix-core stages, rp = cb = 1, 256 <= n <= 8192).
+// (split-BF16 matrix-core stages, rp = cb = 1, 256 <= n <= 8192); it > 1:
+// it row pairs / columns per block in sequence with the next one's inputs
+// prefetched (rp = cb = 1, pm_fft_pipe.hip).
 struct FftPlan {
-    int rp = 2, cb = 1, radix = 4;
+    int rp = 2, cb = 1, radix = 4, it = 1;
 };
 bool plan_valid(int n, FftPlan p);
+bool pipe_plan_compiled(int n, int radix, int it);
+hipError_t row_r2c_pipe(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s);
+hipError_t col_conv_pipe(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
+                         hipStream_t s);
+hipError_t row_c2r_pipe(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
+                        hipStream_t s);
 FftPlan default_plan(int n);
 hipError_t row_r2c(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s);
 hipError_t col_conv(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,

@@ -434,7 +434,7 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members) {
         const pm_sim* h = members[i]->sim;
         auto ng = std::find_if(P.ngroups.begin(), P.ngroups.end(), [&](const BatchPlan::NGroup& g) {
             return g.n == h->cfg.n_lattice && g.plan.rp == h->plan.rp && g.plan.cb == h->plan.cb &&
-                   g.plan.radix == h->plan.radix;
+                   g.plan.radix == h->plan.radix && g.plan.it == h->plan.it;
         });
         if (ng == P.ngroups.end()) P.ngroups.push_back({h->cfg.n_lattice, h->plan, {i}, {0, 0}});
         else ng->idx.push_back(i);

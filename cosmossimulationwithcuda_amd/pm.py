@@ -222,7 +222,7 @@ class Simulation:
     def autotune(self, fft=True, particles=True, reps=5):
         flags = (TUNE_FFT if fft else 0) | (TUNE_PARTICLES if particles else 0)
         _check(lib().pm_autotune(self._h, flags, reps), self._h)
-        return {k: self.get_param(k) for k in ("fft_rp", "fft_cb", "fft_radix", "tile_log", "deposit_chunk",
+        return {k: self.get_param(k) for k in ("fft_rp", "fft_cb", "fft_radix", "fft_it", "tile_log", "deposit_chunk",
                                                  "resort_interval")}
 
     def info(self):

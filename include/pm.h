@@ -185,7 +185,11 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * is split into at most this many equal blocks, default 64, 0 = no cap), "tile_log" (bin edge 2^5 or
  * 2^6 cells; not below the lattice's default) change speed, never results;
  * the FFT plan "fft_rp", "fft_cb", "fft_radix" changes FP32 rounding only
- * (see pm_autotune).  pm_get_param also reads "adaptive_sorts" (early
+ * (see pm_autotune); "fft_it" (row pairs / columns per FFT-pass block,
+ * handled in sequence with the next one's inputs prefetched: 1, or 2 or 4 with
+ * fft_rp = fft_cb = 1 at N = 1024 radix 4 and N = 2048, 4096 radix 8; default
+ * 2 at N = 2048, else 1; setting another batch or radix resets it to 1 where
+ * needed) changes speed only.  pm_get_param also reads "adaptive_sorts" (early
  * re-bins so far).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
@@ -193,7 +197,8 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);
 
 /* Autotuner (no counterpart in the reference).  PM_TUNE_FFT times every
  * compiled FFT plan ("fft_rp" row pairs per row-pass block in {1,2,4},
- * "fft_cb" columns per column-pass block in {1,2,4,8}, "fft_radix" 4 or 8) on
+ * "fft_cb" columns per column-pass block in {1,2,4,8}, "fft_radix" 4 or 8,
+ * "fft_it" 1, 2 or 4) on
  * the current mass lattice and keeps the fastest; PM_TUNE_PARTICLES times
  * tile size, chunk size and re-sort interval over `reps`-step trials on a
  * device snapshot of the state, which is restored afterwards.  The particle

@@ -465,6 +465,59 @@ def test_fft_plans_vs_fp64(n, radix):
                 assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD
 
 
+@pytest.mark.parametrize("n,radix", [(1024, 4), (2048, 8), (4096, 8)])
+@pytest.mark.parametrize("force_mode", [pm.FORCE_SPECTRAL, pm.FORCE_STENCIL])
+def test_fft_prefetch_plans_identical(n, radix, force_mode):
+    """fft_it = 2 and 4 (two or four row pairs / columns per FFT-pass block, the
+    next one's inputs prefetched; the C2R pass batches potential and force
+    rows) give bit-identical potential and force to one per block, and reach
+    the FP64 bar."""
+    npart = 200000
+    lo, hi = n // 4, 3 * n // 4
+    x, y, vx, vy, m = ics.uniform_square(npart, lo, hi, seed=7)
+    out = {}
+    for it in (1, 2, 4):
+        with make_sim(n, npart, force_mode=force_mode) as s:
+            s.set_param("fft_it", 1)
+            s.set_param("fft_radix", radix)
+            s.set_param("fft_rp", 1)
+            s.set_param("fft_cb", 1)
+            s.set_param("fft_it", it)
+            assert s.get_param("fft_it") == it
+            s.load(x, y, vx, vy, m)
+            s.stage(pm.STAGE_DEPOSIT)
+            s.stage(pm.STAGE_FIELD)
+            out[it] = (s.read_lattice(pm.LATTICE_POTENTIAL), s.read_lattice(pm.LATTICE_FORCE),
+                       s.read_lattice(pm.LATTICE_MASS), s.info().mass_shift)
+    for it in (2, 4):
+        assert np.array_equal(out[it][0].view(np.uint32), out[1][0].view(np.uint32))
+        assert np.array_equal(out[it][1].view(np.uint32), out[1][1].view(np.uint32))
+    if n <= 2048:
+        T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
+        assert rel_l2(out[4][0], orc.potential(n, T, out[4][3], out[4][2])) <= REL_L2_FIELD
+
+
+def test_fft_prefetch_plan_rules():
+    with make_sim(1024, 1000) as s:
+        s.set_param("fft_rp", 1)
+        s.set_param("fft_cb", 1)
+        s.set_param("fft_radix", 4)
+        s.set_param("fft_it", 2)
+        s.set_param("fft_rp", 2)                  # no prefetching variant with two row pairs per block:
+        assert s.get_param("fft_it") == 1         # the depth falls back to 1
+        with pytest.raises(pm.PMError):
+            s.set_param("fft_it", 2)              # and cannot be raised while rp = 2
+        s.set_param("fft_rp", 1)
+        for v in (3, 8):
+            with pytest.raises(pm.PMError):
+                s.set_param("fft_it", v)
+    with make_sim(2048, 1000) as s:
+        assert s.get_param("fft_it") == 2         # the default at 2048
+    with make_sim(512, 1000) as s:
+        with pytest.raises(pm.PMError):
+            s.set_param("fft_it", 2)              # not compiled at 512
+
+
 def test_autotune_keeps_state():
     """The autotuner leaves the simulation state untouched: after tuning the
     particle parameters, 5 more steps are bit-identical to an untuned run;

@@ -383,6 +383,10 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
     }
     h->np = n;
     h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
+    if (h->chunk_auto && pmk::auto_chunk(n) != h->bins.chunk) {
+        h->bins.chunk = pmk::auto_chunk(n);
+        h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
+    }
     PM_HIP(h, pmk::iota_ids(h->cur.id, n, h->stream));
     h->rho_valid = false;
     h->field_parity = -1;
@@ -829,6 +833,7 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         // chunk starts depend on the chunk size: re-bin now so every chunk-mapped
         // kernel (deposit, fused push + deposit) sees a consistent map
         h->bins.chunk = (int)value;
+        h->chunk_auto = false;
         h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
         PM_HIP(h, hipSetDevice(h->cfg.device));
         return sort_now(h, false);
@@ -950,7 +955,7 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         const int tl_default = pmk::tile_log_for(n);
         for (int tl : {5, 6}) {
             if (tl < tl_default) continue;
-            for (int chunk : {8192, 12288, 16384, 32768}) {
+            for (int chunk : {8192, 12288, 16384, 32768, 65536, 131072}) {
                 for (int resort : {4, 8, 12}) {
                     h->bins.tl = tl;
                     h->nbins = pmk::bin_count_of(n, tl);
@@ -990,6 +995,7 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         h->bins.tl = best_tl;
         h->nbins = pmk::bin_count_of(n, best_tl);
         h->bins.chunk = best_chunk;
+        h->chunk_auto = false;
         h->max_chunks = pmk::deposit_max_chunks(n, best_tl, h->cap, best_chunk);
         h->resort_interval = best_resort;
         // the bin tables must match the restored particle order and the chosen tiles
@@ -1171,7 +1177,10 @@ pm_status pm_checkpoint_restore(const void* src, int64_t bytes, int32_t device, 
     if (!pmk::plan_valid(c.n_lattice, h->plan)) h->plan = plan_for(c);
     h->resort_interval = hd.resort_interval > 0 ? hd.resort_interval : 8;
     h->resort_permille = hd.resort_permille;
-    h->bins.chunk = hd.deposit_chunk > 0 ? hd.deposit_chunk : h->bins.chunk;
+    if (hd.deposit_chunk > 0) {
+        h->bins.chunk = hd.deposit_chunk;
+        h->chunk_auto = false;
+    }
     h->max_chunks = pmk::deposit_max_chunks(c.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
     if (hd.tile_log != h->bins.tl && pm_set_param(h, "tile_log", hd.tile_log) != PM_OK) {
         pm_destroy(h);

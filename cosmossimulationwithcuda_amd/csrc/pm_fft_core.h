@@ -43,17 +43,6 @@ __device__ __forceinline__ GField gfield(const FieldDesc& d) {
                   (gptr<cplx>)d.force, d.inv_scale, d.gravity};
 }
 
-// Workgroup barrier that orders LDS only.  __syncthreads() is a full
-// workgroup fence: before every barrier it waits for all of the thread's
-// outstanding global loads and stores (vmcnt(0)), so a transform's stage
-// barriers would stall on prefetched inputs and on earlier output stores.
-// The passes exchange data between threads through LDS only, so an LDS-scope
-// fence (lgkmcnt(0)) around s_barrier is all they need.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }

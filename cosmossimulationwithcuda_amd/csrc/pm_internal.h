@@ -15,6 +15,18 @@ constexpr int kBlock = 256;   // threads per block for every streaming kernel (4
 constexpr int kHalo = 7;
 constexpr int kDepositBlock = 512;
 constexpr int kDepositChunkDefault = 12288;          // particles per particle-kernel block (tunable)
+// Automatic chunk: about kChunkTargetBlocks blocks per step-kernel launch,
+// within [kDepositChunkDefault, kDepositChunkMax], a multiple of 4096.
+// Measured: 12288 is fastest at 1e8 particles (2048^2 and 4096^2); at 1e9
+// (2048^2, dense disk cores) larger chunks win (CIC step 18.1 ms at 12288,
+// 16.7 at 65536, 15.3 at 131072): fewer blocks flush their deposit windows
+// into the same dense cells.
+constexpr int kChunkTargetBlocks = 8192;
+constexpr int kDepositChunkMax = 131072;
+inline int auto_chunk(int64_t np) {
+    const int64_t c = (np / kChunkTargetBlocks + 4095) / 4096 * 4096;
+    return (int)(c < kDepositChunkDefault ? kDepositChunkDefault : c > kDepositChunkMax ? kDepositChunkMax : c);
+}
 
 inline int tile_log_for(int n) { return n <= 2048 ? 5 : 6; }
 template <int TL>
@@ -22,6 +34,18 @@ struct TileGeom {
     static constexpr int kEdge = 1 << TL;
     static constexpr int kWin = kEdge + 2 * kHalo + 1;
 };
+
+// Workgroup barrier that orders LDS only.  __syncthreads() is a full
+// workgroup fence: before every barrier it waits for all of the thread's
+// outstanding global loads and stores (vmcnt(0)).  Kernels whose threads
+// exchange data through LDS only (FFT stages, the step kernels' count / rank /
+// window phases) use an LDS-scope fence around s_barrier instead, so
+// prefetched loads and earlier stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Bin of a position: its tile when in bounds (1 <= x, y < N-1, the
 // reference's rule, cuh:257, 497), else the parked bin (N >> tl)^2.
@@ -37,7 +61,9 @@ struct ParticleArrays {
     uint32_t* id;   // load index of the particle in each slot
 };
 
-constexpr int kMaxChunksPerBinDefault = 64;
+// 0 = no cap: measured as fast or faster than a cap of 64 at 1e8 (2048^2
+// disks, 4096^2 Plummer spheres) and, with the automatic chunk, at 1e9.
+constexpr int kMaxChunksPerBinDefault = 0;
 struct BinState {
     int* count;          // [nbins + 1] particles per bin (last = parked)
     int64_t* start;      // [nbins + 2] exclusive prefix of count

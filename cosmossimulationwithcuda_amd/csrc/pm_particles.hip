@@ -152,6 +152,25 @@ __device__ __forceinline__ void deposit_global(unsigned long long* __restrict__ 
     atomicAdd(g + n + 1, c.c11);
 }
 
+// cnt[b] += 1 for every active lane (b >= 0); returns the value before this
+// lane's increment.  Lanes in one bin get consecutive values, so a block's
+// values for a bin are a permutation of 0 .. count - 1.  The lanes that share
+// the wave's first active bin are served by one LDS atomic (in a dense tile
+// nearly every lane of a wave is in the same bin, where per-lane atomics to
+// one address serialise); the others add individually.
+__device__ __forceinline__ int wave_count_add(int* cnt, int b) {
+    const unsigned long long active = __ballot(1);
+    const int b0 = __shfl(b, __ffsll((long long)active) - 1, 64);
+    const unsigned long long same = __ballot(b == b0);
+    if (b != b0) return atomicAdd(&cnt[b], 1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)same) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[b0], __popcll(same));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(same & ((1ull << lane) - 1ull));
+}
+
 // Block c -> (bin b, slots [s0, s1)): chunk_start is the exclusive prefix of
 // chunks over the in-bounds bins, chunk_start[nin] their total; the k chunks
 // of a bin of P particles split it into equal parts [j P/k, (j+1) P/k).
@@ -266,7 +285,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     if (COUNT)
         for (int e = threadIdx.x; e <= nin; e += BLK) hist[e] = 0;
     if (threadIdx.x == 0) s_mig = 0;
-    __syncthreads();
+    lds_barrier();
     const float fhi = (float)(n - 1);
     // Aligned groups of 4 slots: [q0, q1) covers [s0, s1); slots outside it
     // belong to neighbouring chunks and are neither used nor written.  Reads of
@@ -382,7 +401,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
             for (int k = 0; k < 4; ++k) {
                 spill[u][k] = false;
                 if (!ok[u][k]) continue;
-                if (COUNT) atomicAdd(&hist[bin_of(px[u][k], py[u][k], n, TL)], 1);
+                if (COUNT) (void)wave_count_add(hist, bin_of(px[u][k], py[u][k], n, TL));
                 spill[u][k] = in_bounds(px[u][k], py[u][k], fhi) &&
                               !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale);
             }
@@ -394,7 +413,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
                 if (spill[u][k]) { deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale); ++mig; }
     }
     if (mig) atomicAdd(&s_mig, mig);
-    __syncthreads();
+    lds_barrier();
     flush_window<W, BLK>(win, wx0, wy0, rho, n);
     if (COUNT)
         for (int e = threadIdx.x; e <= nin; e += BLK)
@@ -437,7 +456,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
     }
     for (int e = threadIdx.x; e <= nin; e += BLK) lcnt[e] = 0;
     if (threadIdx.x == 0) s_mig = 0;
-    __syncthreads();
+    lds_barrier();
     const float fhi = (float)(n - 1);
     const int64_t q0 = s0 >> 2, q1 = (s1 + 3) >> 2;
     const int nq = (int)(q1 - q0);
@@ -482,7 +501,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
                 bn[u][k] = -1;
                 if (ok[u][k]) {
                     bn[u][k] = bin_of(px[u][k], py[u][k], n, TL);    // bin of the start position
-                    rk[u][k] = atomicAdd(&lcnt[bn[u][k]], 1);
+                    rk[u][k] = wave_count_add(lcnt, bn[u][k]);
                 }
             }
         }
@@ -516,13 +535,13 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (spill[u][k]) { deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale); ++mig; }
-        __syncthreads();                                   // ranks of this pass complete
+        lds_barrier();                                   // ranks of this pass complete
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (bn[u][k] >= 0 && rk[u][k] == 0) lbase[bn[u][k]] = atomicAdd(&next_cursor[bn[u][k]], lcnt[bn[u][k]]);
-        __syncthreads();                                   // bases known
+        lds_barrier();                                   // bases known
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -536,16 +555,16 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
                 dst.m[sl] = pm[u][k];
                 dst.id[sl] = pid[u][k];
             }
-        __syncthreads();                                   // counts read: reset for the next pass
+        lds_barrier();                                   // counts read: reset for the next pass
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (bn[u][k] >= 0 && rk[u][k] == 0) lcnt[bn[u][k]] = 0;
-        __syncthreads();
+        lds_barrier();
     }
     if (mig) atomicAdd(&s_mig, mig);
-    __syncthreads();
+    lds_barrier();
     flush_window<W, BLK>(win, wx0, wy0, rho, n);
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }

@@ -36,6 +36,7 @@ struct pm_sim {
     bool scatter_pending = false;          // the next push writes the re-sorted order
     bool resort_request = false;           // adaptive re-sort: start the fused cycle at the next push
     int fused_sort = 1;                    // re-sort inside the step kernel when it fits
+    bool chunk_auto = true;                // deposit chunk follows the particle count (auto_chunk) at load
     int nbins = 0;                         // in-bounds bins + parked bin
     int nbins_alloc = 0;                   // bins the bin arrays were sized for
     int max_chunks = 0;

@@ -181,8 +181,10 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * (steps between particle re-binnings), "resort_migrants_permille" (re-bin
  * early when more than this many per thousand particles per step land outside
  * their tile's window; 0 = off, default 10), "deposit_chunk" (particles per
- * particle-kernel block, a multiple of 512), "bin_max_chunks" (a dense tile
- * is split into at most this many equal blocks, default 64, 0 = no cap), "tile_log" (bin edge 2^5 or
+ * particle-kernel block, a multiple of 512; by default set at each load to
+ * about n_particles / 8192 within [12288, 131072]; setting it fixes it),
+ * "bin_max_chunks" (a dense tile is split into at most this many equal
+ * blocks, default 0 = no cap), "tile_log" (bin edge 2^5 or
  * 2^6 cells; not below the lattice's default) change speed, never results;
  * the FFT plan "fft_rp", "fft_cb", "fft_radix" changes FP32 rounding only
  * (see pm_autotune); "fft_it" (row pairs / columns per FFT-pass block,

@@ -57,6 +57,8 @@ def algorithmic_bytes(kernel, n, np_):
         "push_deposit": 36 * np_ + 16 * n * n,
         # the same step with the re-sort fused in (moving m and the load index is sort traffic)
         "push_deposit_sort": 36 * np_ + 16 * n * n,
+        # the same step kernel on the step that also counts the next re-sort's bins
+        "push_deposit_count": 36 * np_ + 16 * n * n,
     }[kernel]
 
 

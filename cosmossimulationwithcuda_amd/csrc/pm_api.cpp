@@ -334,7 +334,7 @@ pm_status stage_push_deposit(pm_sim* h) {
         if (due) PM_HIP(h, pmk::fill_zero(count, sizeof(int) * (size_t)h->nbins, h->stream));
     }
     {
-        KernelTimer kt(h, PM_K_PUSH_DEPOSIT);
+        KernelTimer kt(h, due ? PM_K_PUSH_DEPOSIT_COUNT : PM_K_PUSH_DEPOSIT);
         pmk::BinState bs = h->bins;
         bs.count = count;
         const bool pipelined = h->np < (int64_t)pmk::kPipelineMaxDensity * n * n;

@@ -23,7 +23,7 @@ LATTICE_MASS, LATTICE_POTENTIAL, LATTICE_FORCE, LATTICE_TRANSFER = range(4)
 STAGE_DEPOSIT, STAGE_FIELD, STAGE_PUSH = range(3)
 TUNE_FFT, TUNE_PARTICLES = 1, 2
 KERNELS = ["clear", "deposit", "row_r2c", "col_conv", "row_c2r", "edge", "stencil", "push", "sort", "push_deposit",
-           "push_deposit_sort"]
+           "push_deposit_sort", "push_deposit_count"]
 
 
 class PMError(RuntimeError):

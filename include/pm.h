@@ -59,8 +59,11 @@ enum { PM_STAGE_DEPOSIT = 0, PM_STAGE_FIELD = 1, PM_STAGE_PUSH = 2 };
 enum {
     PM_K_CLEAR = 0, PM_K_DEPOSIT = 1, PM_K_ROW_R2C = 2, PM_K_COL_CONV = 3, PM_K_ROW_C2R = 4,
     PM_K_EDGE = 5, PM_K_STENCIL = 6, PM_K_PUSH = 7, PM_K_SORT = 8, PM_K_PUSH_DEPOSIT = 9,
-    PM_K_PUSH_DEPOSIT_SORT = 10, PM_K_COUNT = 11
+    PM_K_PUSH_DEPOSIT_SORT = 10, PM_K_PUSH_DEPOSIT_COUNT = 11, PM_K_COUNT = 12
 };
+/* PM_K_PUSH_DEPOSIT: the steady-state step kernel; PM_K_PUSH_DEPOSIT_COUNT: the
+ * same kernel on the step that also counts the new bins for the next re-sort;
+ * PM_K_PUSH_DEPOSIT_SORT: the step that writes the re-sorted order. */
 
 /* Replaces the compile-time namespace Constants (cuh:29-74, main.cu:5-25) and
  * the ctor arguments (cuh:881) with runtime fields. */

@@ -518,6 +518,18 @@ def test_fft_prefetch_plan_rules():
             s.set_param("fft_it", 2)              # not compiled at 512
 
 
+def test_deposit_chunk_follows_particle_count():
+    n, npart = 256, 300000
+    x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=3)
+    with make_sim(n, npart) as s:
+        assert s.get_param("bin_max_chunks") == 0           # no per-tile cap by default
+        s.load(x, y, vx, vy, m)
+        assert s.get_param("deposit_chunk") == 12288        # the floor (below ~1e8 particles)
+        s.set_param("deposit_chunk", 8192)
+        s.load(x, y, vx, vy, m)
+        assert s.get_param("deposit_chunk") == 8192         # set explicitly: kept across loads
+
+
 def test_autotune_keeps_state():
     """The autotuner leaves the simulation state untouched: after tuning the
     particle parameters, 5 more steps are bit-identical to an untuned run;

@@ -91,7 +91,7 @@ Layout layout_of(const pm_config* c) {
     L.force = align256(8 * n * n);
     const int64_t dblocks = std::max(pmk::diag_blocks(cap), pmk::diag_lattice_blocks((int)n));
     L.diag = align256(8 * 8 * dblocks);
-    L.fdesc = align256(2 * sizeof(pmk::FieldDesc));
+    L.fdesc = align256(4 * sizeof(pmk::FieldDesc));
     L.total = 12 * L.particles + 2 * L.bins + 2 * L.rho + L.fdesc + 2 * L.H + L.T + L.dk + L.tw + L.phi + L.force +
               L.diag;
     L.setup_work = 16 * n * n;
@@ -160,9 +160,10 @@ pm_status resolve_profile(pm_sim* h) {
     return PM_OK;
 }
 
-// Device descriptors of the field passes, one per mass-lattice parity.
+// Device descriptors of the field passes, one per mass-lattice parity; [2 + p]
+// also has the row R2C pass zero the other lattice (the next deposit target).
 pm_status upload_field_desc(pm_sim* h) {
-    pmk::FieldDesc d[2];
+    pmk::FieldDesc d[4];
     for (int p = 0; p < 2; ++p) {
         d[p].rho = h->rho_buf[p];
         d[p].H = h->H;
@@ -173,6 +174,8 @@ pm_status upload_field_desc(pm_sim* h) {
         d[p].force = h->force;
         d[p].inv_scale = std::ldexp(1.0f, -h->shift);
         d[p].gravity = h->cfg.gravity;
+        d[2 + p] = d[p];
+        d[2 + p].rho_zero = h->rho_buf[p ^ 1];
     }
     PM_HIP(h, hipMemcpyAsync(h->fdesc, d, sizeof(d), hipMemcpyHostToDevice, h->stream));
     PM_HIP(h, hipStreamSynchronize(h->stream));
@@ -210,10 +213,12 @@ pm_status stage_deposit(pm_sim* h) {
     return PM_OK;
 }
 
-pm_status stage_field(pm_sim* h) {
+// clear_next: the row R2C pass also zeroes the other mass lattice, which the
+// push that follows deposits into (pm_step; the push then skips its clear).
+pm_status stage_field(pm_sim* h, bool clear_next) {
     const int n = h->cfg.n_lattice;
     const bool spectral = h->cfg.force_mode == PM_FORCE_SPECTRAL;
-    const pmk::FieldDesc* d = h->fdesc + h->rho_cur;
+    const pmk::FieldDesc* d = h->fdesc + h->rho_cur + (clear_next ? 2 : 0);
     {
         KernelTimer kt(h, PM_K_ROW_R2C);
         PM_HIP(h, pmk::row_r2c(n, d, 1, h->tw, h->plan, h->stream));
@@ -286,7 +291,8 @@ pm_status adaptive_resort(pm_sim* h, bool fused) {
     return PM_OK;
 }
 
-pm_status stage_push_deposit(pm_sim* h) {
+// next_cleared: the field stage just zeroed the deposit target.
+pm_status stage_push_deposit(pm_sim* h, bool next_cleared) {
     const int n = h->cfg.n_lattice;
     const bool adaptive = h->resort_permille > 0 && h->mig_host && !capturing(h);
     const bool fused = h->fused_sort && h->resort_interval >= 3 && pmk::scatter_fits(n, h->bins.tl);
@@ -301,7 +307,7 @@ pm_status stage_push_deposit(pm_sim* h) {
     // separate sort pass.  Otherwise the count step is followed by the
     // separate counting sort.
     if (h->scatter_pending && fused) {
-        {
+        if (!next_cleared) {
             KernelTimer kt(h, PM_K_CLEAR);
             PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur ^ 1], (size_t)8 * n * n, h->stream));
         }
@@ -328,9 +334,9 @@ pm_status stage_push_deposit(pm_sim* h) {
     const bool due = (fused && h->resort_request) || h->steps_since_sort + (fused ? 2 : 1) >= h->resort_interval;
     h->resort_request = false;
     int* count = fused ? h->nx_count : h->bins.count;
-    {
+    if (!next_cleared || due) {
         KernelTimer kt(h, PM_K_CLEAR);
-        PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur ^ 1], (size_t)8 * n * n, h->stream));
+        if (!next_cleared) PM_HIP(h, pmk::fill_zero(h->rho_buf[h->rho_cur ^ 1], (size_t)8 * n * n, h->stream));
         if (due) PM_HIP(h, pmk::fill_zero(count, sizeof(int) * (size_t)h->nbins, h->stream));
     }
     {
@@ -676,8 +682,8 @@ pm_status pm_step(pm_handle h, int32_t nsteps) {
     for (int32_t s = 0; s < nsteps; ++s) {
         pm_status st;
         if (!h->rho_valid && (st = stage_deposit(h)) != PM_OK) return st;
-        if ((st = stage_field(h)) != PM_OK) return st;
-        if ((st = stage_push_deposit(h)) != PM_OK) return st;
+        if ((st = stage_field(h, true)) != PM_OK) return st;
+        if ((st = stage_push_deposit(h, true)) != PM_OK) return st;
     }
     return PM_OK;
 }

@@ -53,6 +53,10 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
         const int f = e / N, i = e - f * N;
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
         buf[pad(e)] = make_float2((float)d.rho[ra] * d.inv_scale, (float)d.rho[ra + N] * d.inv_scale);
+        if (d.rho_zero) {
+            d.rho_zero[ra] = 0;
+            d.rho_zero[ra + N] = 0;
+        }
     }
     lds_barrier();
     lds_fft<N, RP, RADIX, false>(buf, twl);

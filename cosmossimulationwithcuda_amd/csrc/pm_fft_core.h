@@ -32,6 +32,7 @@ __device__ __forceinline__ cplx nv(float2 v) { return cplx{v.x, v.y}; }
 
 struct GField {
     gptr<const int64_t> rho;
+    gptr<int64_t> rho_zero;
     gptr<cplx> H, Hfy;
     gptr<const float> T, dk;
     gptr<float> phi;
@@ -39,7 +40,7 @@ struct GField {
     float inv_scale, gravity;
 };
 __device__ __forceinline__ GField gfield(const FieldDesc& d) {
-    return GField{glob(d.rho), (gptr<cplx>)d.H, (gptr<cplx>)d.Hfy, glob(d.T), glob(d.dk), glob(d.phi),
+    return GField{glob(d.rho), glob(d.rho_zero), (gptr<cplx>)d.H, (gptr<cplx>)d.Hfy, glob(d.T), glob(d.dk), glob(d.phi),
                   (gptr<cplx>)d.force, d.inv_scale, d.gravity};
 }
 

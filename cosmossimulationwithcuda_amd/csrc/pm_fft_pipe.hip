@@ -39,6 +39,14 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c_it(const FieldDesc* __restri
 #pragma unroll
         for (int p = 0; p < PER; ++p)
             buf[pad(t + p * kBlock)] = make_float2((float)ra[p] * d.inv_scale, (float)rb[p] * d.inv_scale);
+        if (d.rho_zero) {
+            const gptr<int64_t> z = d.rho_zero + (size_t)j0 * N + t;
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                z[p * kBlock] = 0;
+                z[N + p * kBlock] = 0;
+            }
+        }
         lds_barrier();
         if (it + 1 < IT) fetch(g0 + it + 1);    // in flight during the transform
         lds_fft<N, 1, RADIX, false>(buf, twl);

@@ -137,6 +137,8 @@ struct FieldDesc {
     float2* force;        // (Fx, Fy)
     float inv_scale;      // 2^-shift
     float gravity;        // force multiplier
+    int64_t* rho_zero = nullptr;   // if set, the row R2C pass zeroes its rows of this
+                                   // lattice (the next deposit target) as it reads rho
 };
 // Tile batch and radix of the three passes: rp row pairs per row-pass block,
 // cb columns per column-pass block, radix 4 or 8 (FP32 Stockham stages) or 16

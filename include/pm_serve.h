@@ -43,9 +43,10 @@ typedef struct pm_server_config {
     int32_t mode;               /* PM_SERVE_* */
     int32_t round_steps;        /* steps per round (even; default 8) */
     int32_t use_graphs;         /* replay rounds from captured HIP graphs */
-    int32_t batch_max_lattice;  /* lattices up to this size ... */
+    int32_t batch_max_lattice;  /* lattices up to this size (default 2048) ... */
     int32_t device;
-    int32_t batch_max_particles; /* ... with at most this many particles run in batched launches (BATCHED) */
+    int32_t batch_max_particles; /* ... with at most this many particles (default 131072) run in
+                                  * batched launches (BATCHED) */
     int32_t reserved[7];
 } pm_server_config;
 

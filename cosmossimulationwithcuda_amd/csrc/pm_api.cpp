@@ -233,8 +233,10 @@ pm_status stage_field(pm_sim* h, bool clear_next) {
         PM_HIP(h, pmk::row_c2r(n, d, 1, h->tw, h->plan, spectral, h->stream));
     }
     if (spectral) {
-        KernelTimer kt(h, PM_K_EDGE);
-        PM_HIP(h, pmk::edge_force(n, d, 1, h->stream));
+        if (!pmk::c2r_does_edges(h->plan)) {
+            KernelTimer kt(h, PM_K_EDGE);
+            PM_HIP(h, pmk::edge_force(n, d, 1, h->stream));
+        }
     } else {
         KernelTimer kt(h, PM_K_STENCIL);
         PM_HIP(h, pmk::stencil_force(n, d, 1, h->stream));

@@ -243,10 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
 
 // ------------------------------------------------------------------ stencils --
 // Reference rule (cuh:174-229): F = ((-P[+2] + 8 P[+1]) - 8 P[-1]) + P[-2], /12, *G;
-// neighbours outside [0,N) take the centre value.
-__device__ __forceinline__ float stencil5(float rr, float r, float l, float ll) {
-    return ((-rr + 8.0f * r) - 8.0f * l) + ll;
-}
+// neighbours outside [0,N) take the centre value (stencil5 in pm_fft_core.h).
 
 template <int N>
 __device__ __forceinline__ float grad_x(const float* __restrict__ phi, int i, int j) {

@@ -48,6 +48,13 @@ __device__ __forceinline__ GField gfield(const FieldDesc& d) {
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
 
+// The reference's 4th-order stencil (cuh:174-229) on the FP32 potential:
+// ((-P[+2] + 8 P[+1]) - 8 P[-1]) + P[-2]; neighbours outside [0, N) take the
+// centre value; the force is this / 12 * G.
+__device__ __forceinline__ float stencil5(float rr, float r, float l, float ll) {
+    return ((-rr + 8.0f * r) - 8.0f * l) + ll;
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }

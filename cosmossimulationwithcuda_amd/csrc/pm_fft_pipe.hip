@@ -213,11 +213,39 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
             d.phi[ra] = z.x;
             d.phi[ra + N] = z.y;
         }
-        if (FORCE)
+        if (FORCE) {
+            // The reference's clamped stencil on the edge bands (what the
+            // separate edge pass does after the one-pair-per-block passes):
+            // Fx on columns 0, 1, N-2, N-1 from the row's potential in LDS; Fy
+            // on rows N-2, N-1 with rows N-4, N-3 from the previous pair (this
+            // block's own stores, read back by the thread that wrote them);
+            // Fy on rows 0, 1 once rows 2, 3 are here (below).
+            auto P = [&](int q, int i) { const float2 z = buf[pad(i)]; return q ? z.y : z.x; };
             for (int e = t; e < 2 * N; e += kBlock) {
-                const int q = e / N, i = e - q * N;
-                d.force[(size_t)(j0 + q) * N + i] = nv(buf[pad(N + e)]);
+                const int q = e / N, i = e - q * N, j = j0 + q;
+                cplx v = nv(buf[pad(N + e)]);
+                if (i < 2 || i >= N - 2) {
+                    const float p0 = P(q, i);
+                    const float l = i >= 1 ? P(q, i - 1) : p0, r = i + 1 < N ? P(q, i + 1) : p0;
+                    const float ll = i >= 2 ? P(q, i - 2) : p0, rr = i + 2 < N ? P(q, i + 2) : p0;
+                    v.x = (stencil5(rr, r, l, ll) / 12.0f) * d.gravity;
+                }
+                if (j >= N - 2) {
+                    const float p0 = P(q, i);
+                    const float up = d.phi[(size_t)(N - 3) * N + i], up2 = d.phi[(size_t)(N - 4) * N + i];
+                    const float t1 = q ? P(0, i) : up, t2 = q ? up : up2;     // P[j-1], P[j-2]
+                    const float b1 = q ? p0 : P(1, i);                         // P[j+1] (clamped)
+                    v.y = (stencil5(p0, b1, t1, t2) / 12.0f) * d.gravity;
+                }
+                d.force[(size_t)j * N + i] = v;
             }
+            if (j0 == 2)                         // rows 0 and 1: P[2], P[3] are this pair
+                for (int i = t; i < N; i += kBlock) {
+                    const float p0 = d.phi[i], p1 = d.phi[(size_t)N + i], p2 = P(0, i), p3 = P(1, i);
+                    ((gptr<float>)(d.force + i))[1] = (stencil5(p2, p1, p0, p0) / 12.0f) * d.gravity;
+                    ((gptr<float>)(d.force + (size_t)N + i))[1] = (stencil5(p3, p2, p0, p1) / 12.0f) * d.gravity;
+                }
+        }
         lds_barrier();                         // buf is rewritten by the next row pair
     }
 }

@@ -529,7 +529,7 @@ pm_status launch_batched_steps(pm_server* s, int steps) {
             if (pmk::row_r2c(g.n, d, cnt, h0->tw, g.plan, s->batch_stream) != hipSuccess ||
                 pmk::col_conv(g.n, d, cnt, h0->tw, g.plan, true, s->batch_stream) != hipSuccess ||
                 pmk::row_c2r(g.n, d, cnt, h0->tw, g.plan, true, s->batch_stream) != hipSuccess ||
-                pmk::edge_force(g.n, d, cnt, s->batch_stream) != hipSuccess)
+                (!pmk::c2r_does_edges(g.plan) && pmk::edge_force(g.n, d, cnt, s->batch_stream) != hipSuccess))
                 return PM_EDEVICE;
         }
         for (auto& g : P.mgroups) {

@@ -133,7 +133,7 @@ hipEvent_t pool_event(pm_sim* h) {
 struct KernelTimer {
     pm_sim* h; int kernel; hipEvent_t s = nullptr;
     KernelTimer(pm_sim* hh, int k) : h(hh), kernel(k) {
-        if (h->profiling && (s = pool_event(h))) (void)hipEventRecord(s, h->stream);
+        if (h->profiling && ((h->prof_mask >> k) & 1) && (s = pool_event(h))) (void)hipEventRecord(s, h->stream);
     }
     ~KernelTimer() {
         if (!s) return;
@@ -830,6 +830,10 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         PM_HIP(h, hipSetDevice(h->cfg.device));
         return sort_now(h, false);
     }
+    if (k == "profile_mask") {
+        h->prof_mask = value;
+        return PM_OK;
+    }
     if (k == "resort_migrants_permille") {
         if (value < 0 || value > 1000) return fail(h, PM_EINVAL, "resort_migrants_permille must be in [0, 1000]");
         h->resort_permille = (int)value;
@@ -885,6 +889,7 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
     else if (k == "fft_cb") *value = h->plan.cb;
     else if (k == "fft_radix") *value = h->plan.radix;
     else if (k == "fft_it") *value = h->plan.it;
+    else if (k == "profile_mask") *value = h->prof_mask;
     else return fail(h, PM_EINVAL, "unknown parameter: " + k);
     return PM_OK;
 }

@@ -74,6 +74,7 @@ struct pm_sim {
     std::vector<void*> allocs;
     std::string err;
     bool profiling = false;
+    int64_t prof_mask = -1;                // kernels timed while profiling (bit = PM_K_* id)
     std::vector<pmi::ProfRec> recs;
     std::vector<hipEvent_t> pool;
     double prof_ms[PM_K_COUNT] = {};

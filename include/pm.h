@@ -186,6 +186,7 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * their tile's window; 0 = off, default 10), "deposit_chunk" (particles per
  * particle-kernel block, a multiple of 512; by default set at each load to
  * about n_particles / 8192 within [12288, 131072]; setting it fixes it),
+ * "profile_mask" (kernels timed while pm_profile is on, bit k = PM_K_* id k),
  * "bin_max_chunks" (a dense tile is split into at most this many equal
  * blocks, default 0 = no cap), "tile_log" (bin edge 2^5 or
  * 2^6 cells; not below the lattice's default) change speed, never results;
@@ -212,7 +213,10 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);
 enum { PM_TUNE_FFT = 1, PM_TUNE_PARTICLES = 2 };
 pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps);
 
-/* Per-kernel device timing with HIP events on the handle's stream. */
+/* Per-kernel device timing with HIP events on the handle's stream.  Each timed
+ * launch adds two event records to the stream (about 5 us per launch on
+ * MI355X); pm_set_param "profile_mask" (bit k = kernel id k, default all)
+ * limits the timing to the kernels of interest. */
 pm_status pm_profile(pm_handle h, int32_t enable);
 pm_status pm_profile_read(pm_handle h, int32_t kernel, double* total_ms, int64_t* launches);
 pm_status pm_profile_reset(pm_handle h);

@@ -44,9 +44,12 @@ def algorithmic_bytes(kernel, n, np_):
     atomics' read-modify-write or re-reads are not counted)."""
     nk = n // 2 + 1
     return {
-        "clear": 8 * n * n,
+        # within pm_step only the bin counts of the next re-sort are cleared
+        # by this timer (the row pass zeroes the next int64 lattice)
+        "clear": 4 * ((n // (32 if n <= 2048 else 64)) ** 2 + 1),
         "deposit": 12 * np_ + 8 * n * n,          # read x, y, m; write int64 lattice
-        "row_r2c": 8 * n * n + 8 * nk * n,        # read int64 lattice, write half spectrum
+        # read int64 lattice, write half spectrum, zero the next int64 lattice
+        "row_r2c": 8 * n * n + 8 * nk * n + 8 * n * n,
         "col_conv": 8 * nk * n + 4 * nk * n + 16 * nk * n,   # read H, read T, write H and Hfy
         "row_c2r": 16 * nk * n + 4 * n * n + 8 * n * n,      # read H, Hfy; write Phi and force
         "edge": 8 * 8 * n,

@@ -22,8 +22,10 @@
 // Fourier symbol of the reference's 4th-order stencil; it reproduces that
 // stencil exactly on every cell whose 5-point neighbourhood lies inside the
 // lattice.  edge_force then recomputes the two outermost rows/columns with the
-// reference's clamped stencil on the FP32 potential.  stencil_force applies that
-// stencil to every cell (force_mode = stencil).
+// reference's clamped stencil on the FP32 potential (the prefetching C2R pass
+// of pm_fft_pipe.hip does this itself).  stencil_force applies that stencil to
+// every cell (force_mode = stencil).  Inside pm_step the row R2C pass also
+// zeroes the other int64 lattice, the next deposit target.
 //
 // FFT: in-place Stockham on LDS, every thread of the block working across all
 // transforms of the block; stages of radix 4 or 8 (plan choice) after one

@@ -43,15 +43,16 @@ namespace {
 
 // ------------------------------------------------------------------ row R2C --
 template <int N, int RP, int RADIX>
-__global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict__ descs,
+__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_r2c(const FieldDesc* __restrict__ descs,
                                                      const float2* __restrict__ tw) {
+    constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded(RP * N)];
     __shared__ float2 twl[N / 4];
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
-    load_twiddles<N>(twl, tw);
-    for (int e = t; e < RP * N; e += kBlock) {
+    load_twiddles<N, BT>(twl, tw);
+    for (int e = t; e < RP * N; e += BT) {
         const int f = e / N, i = e - f * N;
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
         buf[pad(e)] = make_float2((float)d.rho[ra] * d.inv_scale, (float)d.rho[ra + N] * d.inv_scale);
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
     lds_barrier();
     lds_fft<N, RP, RADIX, false>(buf, twl);
     constexpr int NK = N / 2 + 1;
-    for (int e = t; e < NK * RP; e += kBlock) {
+    for (int e = t; e < NK * RP; e += BT) {
         const int k = e / RP, f = e - k * RP;
         const float2 z = buf[pad(f * N + k)];
         const float2 zc = buf[pad(f * N + ((N - k) & (N - 1)))];
@@ -82,22 +83,23 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c(const FieldDesc* __restrict_
 // and W = A*T0 + i*B*TN2 transforms back to the two real results (real and
 // imaginary parts).  N/2 blocks of CB = 1 then fill the device evenly.
 template <int N, int CB, int RADIX, bool FORCE>
-__global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict__ descs,
+__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv(const FieldDesc* __restrict__ descs,
                                                       const float2* __restrict__ tw) {
+    constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded((FORCE ? 2 : 1) * CB * N)];
     __shared__ float2 twl[N / 4];
     const GField d = gfield(descs[blockIdx.y]);
     constexpr int NY = N / 2;                 // the Nyquist column's kx
     const int t = threadIdx.x;
     const int k0 = blockIdx.x * CB;
-    load_twiddles<N>(twl, tw);
-    for (int e = t; e < CB * N; e += kBlock) {
+    load_twiddles<N, BT>(twl, tw);
+    for (int e = t; e < CB * N; e += BT) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
         buf[pad(e)] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : f2(d.H[(size_t)k * N + ky]);
     }
     lds_barrier();
     lds_fft<N, CB, RADIX, false>(buf, twl);
-    for (int e = (k0 == 0 ? N : 0) + t; e < CB * N; e += kBlock) {
+    for (int e = (k0 == 0 ? N : 0) + t; e < CB * N; e += BT) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
         const float tv = d.T[(size_t)k * N + ky];
         const float2 x = buf[pad(e)];
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
         }
     }
     if (k0 == 0) {                            // packed kx = 0 / N/2 slot, by conjugate pairs
-        for (int a = t; a <= N / 2; a += kBlock) {
+        for (int a = t; a <= N / 2; a += BT) {
             const int b = (N - a) & (N - 1);
             const float2 za = buf[pad(a)], zb = buf[pad(b)];
             const float2 A = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
     }
     lds_barrier();
     lds_fft<N, (FORCE ? 2 : 1) * CB, RADIX, true>(buf, twl);
-    for (int e = t; e < CB * N; e += kBlock) {
+    for (int e = t; e < CB * N; e += BT) {
         const int f = e / N, j = e - f * N, k = k0 + f;
         const float2 w = buf[pad(e)];
         const float2 v = FORCE ? buf[pad(CB * N + e)] : make_float2(0.f, 0.f);
@@ -150,8 +152,9 @@ __global__ __launch_bounds__(kBlock) void k_col_conv(const FieldDesc* __restrict
 
 // ------------------------------------------------------------------ row C2R --
 template <int N, int RP, int RADIX, bool FORCE>
-__global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict__ descs,
+__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_c2r(const FieldDesc* __restrict__ descs,
                                                      const float2* __restrict__ tw) {
+    constexpr int BT = fft_threads<N, RADIX>();
     // MERGE: the potential row pairs and the force rows go through one
     // transform batch (one read of the spectra, half the stage barriers).
     // Measured faster up to N = 1024; above, the larger LDS footprint halves
@@ -163,9 +166,9 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
     constexpr int NK = N / 2 + 1;
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
-    load_twiddles<N>(twl, tw);
+    load_twiddles<N, BT>(twl, tw);
     if constexpr (MERGE) {
-        for (int e = t; e < NK * RP; e += kBlock) {
+        for (int e = t; e < NK * RP; e += BT) {
             const int k = e / RP, f = e - k * RP;
             const size_t src = (size_t)k * N + j0 + 2 * f;
             const cplx2 v = *(gptr<const cplx2>)(d.H + src);
@@ -189,21 +192,21 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
         }
         lds_barrier();
         lds_fft<N, 3 * RP, RADIX, true>(buf, twl);
-        for (int e = t; e < RP * N; e += kBlock) {
+        for (int e = t; e < RP * N; e += BT) {
             const int f = e / N, i = e - f * N;
             const size_t ra = (size_t)(j0 + 2 * f) * N + i;
             const float2 z = buf[pad(e)];
             d.phi[ra] = z.x;
             d.phi[ra + N] = z.y;
         }
-        for (int e = t; e < 2 * RP * N; e += kBlock) {
+        for (int e = t; e < 2 * RP * N; e += BT) {
             const int q = e / N, i = e - q * N;
             d.force[(size_t)(j0 + q) * N + i] = nv(buf[pad(RP * N + e)]);
         }
         return;
     }
     // potential: rows (j0+2f, j0+2f+1) packed as one complex row
-    for (int e = t; e < NK * RP; e += kBlock) {
+    for (int e = t; e < NK * RP; e += BT) {
         const int k = e / RP, f = e - k * RP;
         const cplx2 v = *(gptr<const cplx2>)(d.H + (size_t)k * N + j0 + 2 * f);
         float2 xa = make_float2(v.x, v.y), xb = make_float2(v.z, v.w);
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
     }
     lds_barrier();
     lds_fft<N, RP, RADIX, true>(buf, twl);
-    for (int e = t; e < RP * N; e += kBlock) {
+    for (int e = t; e < RP * N; e += BT) {
         const int f = e / N, i = e - f * N;
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
         const float2 z = buf[pad(e)];
@@ -224,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
     lds_barrier();
     // force: one row per complex FFT, Z = Fx + i Fy
     constexpr int R2 = 2 * RP;
-    for (int e = t; e < NK * R2; e += kBlock) {
+    for (int e = t; e < NK * R2; e += BT) {
         const int k = e / R2, q = e - k * R2;
         const size_t src = (size_t)k * N + j0 + q;
         const float2 p = f2(d.H[src]);
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r(const FieldDesc* __restrict_
     }
     lds_barrier();
     lds_fft<N, R2, RADIX, true>(buf, twl);
-    for (int e = t; e < R2 * N; e += kBlock) {
+    for (int e = t; e < R2 * N; e += BT) {
         const int q = e / N, i = e - q * N;
         d.force[(size_t)(j0 + q) * N + i] = nv(buf[pad(e)]);
     }
@@ -295,23 +298,23 @@ __global__ __launch_bounds__(kBlock) void k_stencil(const FieldDesc* __restrict_
 // ------------------------------------------------------------------ launchers --
 template <int N, int RP, int RADIX>
 hipError_t r2c_t(const FieldDesc* d, int count, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_row_r2c<N, RP, RADIX>), dim3(N / (2 * RP), count), dim3(kBlock), 0, s, d, tw);
+    hipLaunchKernelGGL((k_row_r2c<N, RP, RADIX>), dim3(N / (2 * RP), count), dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 
 template <int N, int CB, int RADIX>
 hipError_t col_t(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
     const dim3 grid(N / 2 / CB, count);
-    if (force) hipLaunchKernelGGL((k_col_conv<N, CB, RADIX, true>), grid, dim3(kBlock), 0, s, d, tw);
-    else hipLaunchKernelGGL((k_col_conv<N, CB, RADIX, false>), grid, dim3(kBlock), 0, s, d, tw);
+    if (force) hipLaunchKernelGGL((k_col_conv<N, CB, RADIX, true>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_col_conv<N, CB, RADIX, false>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 
 template <int N, int RP, int RADIX>
 hipError_t c2r_t(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
     const dim3 grid(N / (2 * RP), count);
-    if (force) hipLaunchKernelGGL((k_row_c2r<N, RP, RADIX, true>), grid, dim3(kBlock), 0, s, d, tw);
-    else hipLaunchKernelGGL((k_row_c2r<N, RP, RADIX, false>), grid, dim3(kBlock), 0, s, d, tw);
+    if (force) hipLaunchKernelGGL((k_row_c2r<N, RP, RADIX, true>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_row_c2r<N, RP, RADIX, false>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 
@@ -394,15 +397,17 @@ bool plan_valid(int n, FftPlan p) {
 }
 
 // Measured on MI355X over n = 256..4096: one row pair / one column per block
-// (the most blocks in flight) is fastest at every size; radix 8 from 2048 up;
-// at 2048 two row pairs / columns per block with the next one prefetched
-// take the field solve from 107 to 102 us (slower at 1024 and 4096).
+// (the most blocks in flight) beats larger batches at every size; radix 8 from
+// 2048 up.  Two row pairs / columns per block in sequence, with the next one
+// prefetched and the edge bands done in the C2R pass, take the field solve
+// from 64.7 to 58.6 us at 1024, 107 to 97 us at 2048 and 0.36 to 0.34 ms at
+// 4096 (with 512-thread blocks there); four per block are slower.
 FftPlan default_plan(int n) {
     FftPlan p;
     p.radix = n >= 2048 ? 8 : 4;
     p.rp = 1;
     p.cb = 1;
-    p.it = n == 2048 ? 2 : 1;
+    p.it = pipe_plan_compiled(n, p.radix, 2) ? 2 : 1;
     return p;
 }
 

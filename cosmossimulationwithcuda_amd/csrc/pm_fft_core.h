@@ -151,26 +151,33 @@ __device__ __forceinline__ int row_group() {
     return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Threads per block of the FFT passes: 512 for N >= 4096 with the FP32
+// stages (twice the waves per CU at the same LDS footprint: 4096^2 field
+// solve 0.36-0.41 -> 0.34 ms), else 256 (512 is slower at 2048 with two row
+// pairs per block; the matrix-core radix-16 stages are written for 256).
+template <int N, int RADIX>
+constexpr int fft_threads() { return N >= 4096 && RADIX != 16 ? 512 : kBlock; }
+
 // Loads the quarter twiddle table into LDS (callers synchronise after).
-template <int N>
+template <int N, int BT = kBlock>
 __device__ __forceinline__ void load_twiddles(float2* q, const float2* __restrict__ tw) {
-    for (int e = threadIdx.x; e < N / 4; e += kBlock) q[e] = tw[e];
+    for (int e = threadIdx.x; e < N / 4; e += BT) q[e] = tw[e];
 }
 
 // One Stockham stage of radix R with sub-transform length NS over NFFT
 // in-place transforms of length N at buf + f*N (padded).  Reads, barrier,
 // writes, barrier.
-template <int N, int NFFT, int R, int NS, bool INV>
+template <int N, int NFFT, int R, int NS, bool INV, int BT = kBlock>
 __device__ __forceinline__ void stage(float2* buf, const float2* tw) {
-    constexpr int Q = N / R, NB = NFFT * Q, PER = (NB + kBlock - 1) / kBlock;
+    constexpr int Q = N / R, NB = NFFT * Q, PER = (NB + BT - 1) / BT;
     constexpr int STEP = N / (R * NS);
     const int t = threadIdx.x;
     float2 v[PER][R];
     int ob[PER];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int bb = t + p * kBlock;
-        if (NB % kBlock == 0 || bb < NB) {
+        const int bb = t + p * BT;
+        if (NB % BT == 0 || bb < NB) {
             const int f = bb / Q, b = bb - f * Q;
             if constexpr (Q % 32 == 0) {
                 const float2* src = buf + pad(f * N + b);
@@ -193,8 +200,8 @@ __device__ __forceinline__ void stage(float2* buf, const float2* tw) {
     lds_barrier();
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int bb = t + p * kBlock;
-        if (NB % kBlock == 0 || bb < NB) {
+        const int bb = t + p * BT;
+        if (NB % BT == 0 || bb < NB) {
             if constexpr (NS % 32 == 0) {
                 float2* dst = buf + pad(ob[p]);
 #pragma unroll
@@ -212,11 +219,11 @@ __device__ __forceinline__ void stage(float2* buf, const float2* tw) {
     lds_barrier();
 }
 
-template <int N, int NFFT, int R, int NS, bool INV>
+template <int N, int NFFT, int R, int NS, bool INV, int BT = kBlock>
 __device__ __forceinline__ void stages_from(float2* buf, const float2* tw) {
     if constexpr (NS < N) {
-        stage<N, NFFT, R, NS, INV>(buf, tw);
-        stages_from<N, NFFT, R, NS * R, INV>(buf, tw);
+        stage<N, NFFT, R, NS, INV, BT>(buf, tw);
+        stages_from<N, NFFT, R, NS * R, INV, BT>(buf, tw);
     }
 }
 
@@ -362,17 +369,18 @@ __device__ __forceinline__ void lds_fft_mfma(float2* buf, const float2* tw) {
 
 // RADIX 4 or 8: FP32 Stockham stages; RADIX 16: the split-BF16 matrix-core
 // variant above.
-template <int N, int NFFT, int RADIX, bool INV>
+template <int N, int NFFT, int RADIX, bool INV, int BT = fft_threads<N, RADIX>()>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
     constexpr int LR = ilog2c(RADIX), L = ilog2c(N), REM = L % LR;
     if constexpr (RADIX == 16) {
+        static_assert(BT == kBlock, "the matrix-core stages run 256-thread blocks");
         lds_fft_mfma<N, NFFT, INV>(buf, tw);
     } else if constexpr (REM == 0) {
-        stages_from<N, NFFT, RADIX, 1, INV>(buf, tw);
+        stages_from<N, NFFT, RADIX, 1, INV, BT>(buf, tw);
     } else {
         constexpr int R0 = 1 << REM;
-        stage<N, NFFT, R0, 1, INV>(buf, tw);
-        stages_from<N, NFFT, RADIX, R0, INV>(buf, tw);
+        stage<N, NFFT, R0, 1, INV, BT>(buf, tw);
+        stages_from<N, NFFT, RADIX, R0, INV, BT>(buf, tw);
     }
 }
 

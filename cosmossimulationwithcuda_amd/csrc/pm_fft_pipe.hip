@@ -15,11 +15,12 @@ namespace {
 
 // ------------------------------------------------------------------ row R2C --
 template <int N, int RADIX, int IT>
-__global__ __launch_bounds__(kBlock) void k_row_r2c_it(const FieldDesc* __restrict__ descs,
+__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_r2c_it(const FieldDesc* __restrict__ descs,
                                                         const float2* __restrict__ tw) {
+    constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded(N)];
     __shared__ float2 twl[N / 4];
-    constexpr int PER = N / kBlock, NK = N / 2 + 1;
+    constexpr int PER = N / BT, NK = N / 2 + 1;
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
     const int g0 = row_group() * IT;            // row pairs g0 .. g0 + IT - 1
@@ -28,29 +29,29 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c_it(const FieldDesc* __restri
         const gptr<const int64_t> r = d.rho + (size_t)(2 * g) * N + t;
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
-            ra[p] = r[p * kBlock];
-            rb[p] = r[N + p * kBlock];
+            ra[p] = r[p * BT];
+            rb[p] = r[N + p * BT];
         }
     };
-    load_twiddles<N>(twl, tw);
+    load_twiddles<N, BT>(twl, tw);
     fetch(g0);
     for (int it = 0; it < IT; ++it) {
         const int j0 = 2 * (g0 + it);
 #pragma unroll
         for (int p = 0; p < PER; ++p)
-            buf[pad(t + p * kBlock)] = make_float2((float)ra[p] * d.inv_scale, (float)rb[p] * d.inv_scale);
+            buf[pad(t + p * BT)] = make_float2((float)ra[p] * d.inv_scale, (float)rb[p] * d.inv_scale);
         if (d.rho_zero) {
             const gptr<int64_t> z = d.rho_zero + (size_t)j0 * N + t;
 #pragma unroll
             for (int p = 0; p < PER; ++p) {
-                z[p * kBlock] = 0;
-                z[N + p * kBlock] = 0;
+                z[p * BT] = 0;
+                z[N + p * BT] = 0;
             }
         }
         lds_barrier();
         if (it + 1 < IT) fetch(g0 + it + 1);    // in flight during the transform
         lds_fft<N, 1, RADIX, false>(buf, twl);
-        for (int k = t; k < NK; k += kBlock) {
+        for (int k = t; k < NK; k += BT) {
             const float2 z = buf[pad(k)];
             const float2 zc = buf[pad((N - k) & (N - 1))];
             // Xa = (Z + conj(Zc))/2,  Xb = (Z - conj(Zc))/(2i)
@@ -66,11 +67,12 @@ __global__ __launch_bounds__(kBlock) void k_row_r2c_it(const FieldDesc* __restri
 // Columns kx = blockIdx.x * IT + it; kx = 0 carries the packed kx = 0 / N/2
 // pair exactly as the one-column pass does.
 template <int N, int RADIX, bool FORCE, int IT>
-__global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restrict__ descs,
+__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv_it(const FieldDesc* __restrict__ descs,
                                                          const float2* __restrict__ tw) {
+    constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded((FORCE ? 2 : 1) * N)];
     __shared__ float2 twl[N / 4];
-    constexpr int NY = N / 2, PER = N / kBlock;
+    constexpr int NY = N / 2, PER = N / BT;
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
     const int kb = blockIdx.x * IT;
@@ -79,19 +81,19 @@ __global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restr
     auto fetch = [&](int k) {
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
-            const int ky = t + p * kBlock;
+            const int ky = t + p * BT;
             h[p] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : f2(d.H[(size_t)k * N + ky]);
             tv[p] = d.T[(size_t)k * N + ky];
         }
     };
-    load_twiddles<N>(twl, tw);
+    load_twiddles<N, BT>(twl, tw);
     fetch(kb);
     for (int it = 0; it < IT; ++it) {
         const int k = kb + it;
         float tc[PER];
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
-            buf[pad(t + p * kBlock)] = h[p];
+            buf[pad(t + p * BT)] = h[p];
             tc[p] = tv[p];
         }
         lds_barrier();
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restr
         if (k != 0) {
 #pragma unroll
             for (int p = 0; p < PER; ++p) {
-                const int e = t + p * kBlock;
+                const int e = t + p * BT;
                 const float2 x = buf[pad(e)];
                 const float2 y = make_float2(x.x * tc[p], x.y * tc[p]);
                 buf[pad(e)] = y;
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restr
                 }
             }
         } else {                                 // packed kx = 0 / N/2 slot, by conjugate pairs
-            for (int a = t; a <= N / 2; a += kBlock) {
+            for (int a = t; a <= N / 2; a += BT) {
                 const int b = (N - a) & (N - 1);
                 const float2 za = buf[pad(a)], zb = buf[pad(b)];
                 const float2 A = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restr
         }
         lds_barrier();
         lds_fft<N, FORCE ? 2 : 1, RADIX, true>(buf, twl);
-        for (int j = t; j < N; j += kBlock) {
+        for (int j = t; j < N; j += BT) {
             const float2 w = buf[pad(j)];
             const float2 v = FORCE ? buf[pad(N + j)] : make_float2(0.f, 0.f);
             if (k == 0) {
@@ -155,9 +157,10 @@ __global__ __launch_bounds__(kBlock) void k_col_conv_it(const FieldDesc* __restr
 // FORCE, the force rows (Z = Fx + i Fy per row, Fx-hat = i d(kx) Phi-hat) as
 // one batch of 1 or 3 transforms.
 template <int N, int RADIX, bool FORCE, int IT>
-__global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restrict__ descs,
+__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_c2r_it(const FieldDesc* __restrict__ descs,
                                                         const float2* __restrict__ tw) {
-    constexpr int ROWS = FORCE ? 3 : 1, NK = N / 2 + 1, PK = (NK + kBlock - 1) / kBlock;
+    constexpr int BT = fft_threads<N, RADIX>();
+    constexpr int ROWS = FORCE ? 3 : 1, NK = N / 2 + 1, PK = (NK + BT - 1) / BT;
     __shared__ float2 buf[padded(ROWS * N)];
     __shared__ float2 twl[N / 4];
     const GField d = gfield(descs[blockIdx.y]);
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
     auto fetch = [&](int g) {
 #pragma unroll
         for (int q = 0; q < PK; ++q) {
-            const int k = t + q * kBlock;
+            const int k = t + q * BT;
             if (k < NK) {
                 const size_t src = (size_t)k * N + 2 * g;
                 hv[q] = *(gptr<const cplx2>)(d.H + src);
@@ -175,13 +178,13 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
             }
         }
     };
-    load_twiddles<N>(twl, tw);
+    load_twiddles<N, BT>(twl, tw);
     fetch(g0);
     for (int it = 0; it < IT; ++it) {
         const int j0 = 2 * (g0 + it);
 #pragma unroll
         for (int q = 0; q < PK; ++q) {
-            const int k = t + q * kBlock;
+            const int k = t + q * BT;
             if (k >= NK) continue;
             const cplx2 v = hv[q];
             const bool edge = k == 0 || k == N / 2;
@@ -207,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
         lds_barrier();
         if (it + 1 < IT) fetch(g0 + it + 1);    // in flight during the transforms
         lds_fft<N, ROWS, RADIX, true>(buf, twl);
-        for (int i = t; i < N; i += kBlock) {
+        for (int i = t; i < N; i += BT) {
             const size_t ra = (size_t)j0 * N + i;
             const float2 z = buf[pad(i)];
             d.phi[ra] = z.x;
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
             // block's own stores, read back by the thread that wrote them);
             // Fy on rows 0, 1 once rows 2, 3 are here (below).
             auto P = [&](int q, int i) { const float2 z = buf[pad(i)]; return q ? z.y : z.x; };
-            for (int e = t; e < 2 * N; e += kBlock) {
+            for (int e = t; e < 2 * N; e += BT) {
                 const int q = e / N, i = e - q * N, j = j0 + q;
                 cplx v = nv(buf[pad(N + e)]);
                 if (i < 2 || i >= N - 2) {
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
                 d.force[(size_t)j * N + i] = v;
             }
             if (j0 == 2)                         // rows 0 and 1: P[2], P[3] are this pair
-                for (int i = t; i < N; i += kBlock) {
+                for (int i = t; i < N; i += BT) {
                     const float p0 = d.phi[i], p1 = d.phi[(size_t)N + i], p2 = P(0, i), p3 = P(1, i);
                     ((gptr<float>)(d.force + i))[1] = (stencil5(p2, p1, p0, p0) / 12.0f) * d.gravity;
                     ((gptr<float>)(d.force + (size_t)N + i))[1] = (stencil5(p3, p2, p0, p1) / 12.0f) * d.gravity;
@@ -252,21 +255,21 @@ __global__ __launch_bounds__(kBlock) void k_row_c2r_it(const FieldDesc* __restri
 
 template <int N, int RADIX, int IT>
 hipError_t launch_r2c(const FieldDesc* d, int count, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_row_r2c_it<N, RADIX, IT>), dim3(N / 2 / IT, count), dim3(kBlock), 0, s, d, tw);
+    hipLaunchKernelGGL((k_row_r2c_it<N, RADIX, IT>), dim3(N / 2 / IT, count), dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 template <int N, int RADIX, int IT>
 hipError_t launch_col(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
     const dim3 grid(N / 2 / IT, count);
-    if (force) hipLaunchKernelGGL((k_col_conv_it<N, RADIX, true, IT>), grid, dim3(kBlock), 0, s, d, tw);
-    else hipLaunchKernelGGL((k_col_conv_it<N, RADIX, false, IT>), grid, dim3(kBlock), 0, s, d, tw);
+    if (force) hipLaunchKernelGGL((k_col_conv_it<N, RADIX, true, IT>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_col_conv_it<N, RADIX, false, IT>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 template <int N, int RADIX, int IT>
 hipError_t launch_c2r(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
     const dim3 grid(N / 2 / IT, count);
-    if (force) hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, true, IT>), grid, dim3(kBlock), 0, s, d, tw);
-    else hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, false, IT>), grid, dim3(kBlock), 0, s, d, tw);
+    if (force) hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, true, IT>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, false, IT>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 

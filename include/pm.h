@@ -194,8 +194,8 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * (see pm_autotune); "fft_it" (row pairs / columns per FFT-pass block,
  * handled in sequence with the next one's inputs prefetched: 1, or 2 or 4 with
  * fft_rp = fft_cb = 1 at N = 1024 radix 4 and N = 2048, 4096 radix 8; default
- * 2 at N = 2048, else 1; setting another batch or radix resets it to 1 where
- * needed) changes speed only.  pm_get_param also reads "adaptive_sorts" (early
+ * 2 at those sizes, else 1; setting another batch or radix resets it to 1
+ * where needed) changes speed only.  pm_get_param also reads "adaptive_sorts" (early
  * re-bins so far).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);

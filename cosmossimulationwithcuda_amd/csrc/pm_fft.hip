@@ -401,7 +401,8 @@ bool plan_valid(int n, FftPlan p) {
 // 2048 up.  Two row pairs / columns per block in sequence, with the next one
 // prefetched and the edge bands done in the C2R pass, take the field solve
 // from 64.7 to 58.6 us at 1024, 107 to 97 us at 2048 and 0.36 to 0.34 ms at
-// 4096 (with 512-thread blocks there); four per block are slower.
+// 4096 (with 512-thread blocks there); four per block are slower.  At 256 and
+// 512 they serve batches of simulations (one edge launch fewer per step).
 FftPlan default_plan(int n) {
     FftPlan p;
     p.radix = n >= 2048 ? 8 : 4;

@@ -286,8 +286,8 @@ hipError_t pipe_pass(int pass, int n, const FieldDesc* d, int count, const float
                      hipStream_t s) {
 #define PM_PIPE(NV, RV, ITV) \
     if (n == NV && p.radix == RV && p.it == ITV) return launch_pass<NV, RV, ITV>(pass, d, count, tw, force, s);
-    PM_PIPE(1024, 4, 2) PM_PIPE(1024, 4, 4) PM_PIPE(2048, 8, 2) PM_PIPE(2048, 8, 4) PM_PIPE(4096, 8, 2)
-    PM_PIPE(4096, 8, 4)
+    PM_PIPE(256, 4, 2) PM_PIPE(512, 4, 2) PM_PIPE(1024, 4, 2) PM_PIPE(1024, 4, 4) PM_PIPE(2048, 8, 2)
+    PM_PIPE(2048, 8, 4) PM_PIPE(4096, 8, 2) PM_PIPE(4096, 8, 4)
 #undef PM_PIPE
     return hipErrorInvalidValue;
 }
@@ -295,6 +295,7 @@ hipError_t pipe_pass(int pass, int n, const FieldDesc* d, int count, const float
 }  // namespace
 
 bool pipe_plan_compiled(int n, int radix, int it) {
+    if ((n == 256 || n == 512) && radix == 4) return it == 2;
     if (it != 2 && it != 4) return false;
     return (n == 1024 && radix == 4) || ((n == 2048 || n == 4096) && radix == 8);
 }

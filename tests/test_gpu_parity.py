@@ -465,9 +465,10 @@ def test_fft_plans_vs_fp64(n, radix):
                 assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD
 
 
-@pytest.mark.parametrize("n,radix", [(1024, 4), (2048, 8), (4096, 8)])
+@pytest.mark.parametrize("n,radix,its", [(256, 4, (1, 2)), (512, 4, (1, 2)), (1024, 4, (1, 2, 4)),
+                                          (2048, 8, (1, 2, 4)), (4096, 8, (1, 2, 4))])
 @pytest.mark.parametrize("force_mode", [pm.FORCE_SPECTRAL, pm.FORCE_STENCIL])
-def test_fft_prefetch_plans_identical(n, radix, force_mode):
+def test_fft_prefetch_plans_identical(n, radix, its, force_mode):
     """fft_it = 2 and 4 (two or four row pairs / columns per FFT-pass block, the
     next one's inputs prefetched; the C2R pass batches potential and force
     rows) give bit-identical potential and force to one per block, and reach
@@ -476,7 +477,7 @@ def test_fft_prefetch_plans_identical(n, radix, force_mode):
     lo, hi = n // 4, 3 * n // 4
     x, y, vx, vy, m = ics.uniform_square(npart, lo, hi, seed=7)
     out = {}
-    for it in (1, 2, 4):
+    for it in its:
         with make_sim(n, npart, force_mode=force_mode) as s:
             s.set_param("fft_it", 1)
             s.set_param("fft_radix", radix)
@@ -489,12 +490,13 @@ def test_fft_prefetch_plans_identical(n, radix, force_mode):
             s.stage(pm.STAGE_FIELD)
             out[it] = (s.read_lattice(pm.LATTICE_POTENTIAL), s.read_lattice(pm.LATTICE_FORCE),
                        s.read_lattice(pm.LATTICE_MASS), s.info().mass_shift)
-    for it in (2, 4):
+    for it in its[1:]:
         assert np.array_equal(out[it][0].view(np.uint32), out[1][0].view(np.uint32))
         assert np.array_equal(out[it][1].view(np.uint32), out[1][1].view(np.uint32))
     if n <= 2048:
         T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
-        assert rel_l2(out[4][0], orc.potential(n, T, out[4][3], out[4][2])) <= REL_L2_FIELD
+        last = out[its[-1]]
+        assert rel_l2(last[0], orc.potential(n, T, last[3], last[2])) <= REL_L2_FIELD
 
 
 def test_fft_prefetch_plan_rules():
@@ -511,11 +513,16 @@ def test_fft_prefetch_plan_rules():
         for v in (3, 8):
             with pytest.raises(pm.PMError):
                 s.set_param("fft_it", v)
-    with make_sim(2048, 1000) as s:
-        assert s.get_param("fft_it") == 2         # the default at 2048
+    for n in (256, 512, 1024, 2048, 4096):
+        with make_sim(n, 1000) as s:
+            assert s.get_param("fft_it") == 2     # the default at these sizes
     with make_sim(512, 1000) as s:
         with pytest.raises(pm.PMError):
-            s.set_param("fft_it", 2)              # not compiled at 512
+            s.set_param("fft_it", 4)              # not compiled at 512
+    with make_sim(128, 1000) as s:
+        assert s.get_param("fft_it") == 1
+        with pytest.raises(pm.PMError):
+            s.set_param("fft_it", 2)              # not compiled at 128
 
 
 def test_deposit_chunk_follows_particle_count():

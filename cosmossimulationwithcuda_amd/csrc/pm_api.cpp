@@ -63,6 +63,11 @@ int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
 // FP32: Stockham radix 4/8 (default plan for the size); BF16X3: radix-16
 // matrix-core stages, one row pair / column per block.
+// Re-sort every 7 steps by default: measured 0.927 ms per step on the
+// headline case against 0.931 at 8, 0.930 at 6 and 1.03 at 10 (the re-sort
+// step itself grows with the particles' drift since the last sort).
+constexpr int kResortIntervalDefault = 7;
+
 pmk::FftPlan plan_for(const pm_config& c) {
     if (c.precision == PM_FFT_BF16X3) {
         pmk::FftPlan p;
@@ -455,7 +460,7 @@ pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src) {
     const int64_t cap = h->cap;
     h->cfg = *cfg;
     h->cfg.n_particles = cap;
-    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
+    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : kResortIntervalDefault;
     pm_status st = PM_OK;
     if (spectra && (st = build_spectra(h, T_src)) != PM_OK) return st;
     if (tables && (st = upload_tables(h)) != PM_OK) return st;
@@ -571,7 +576,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
     h->nbins = pmk::bin_count_of(n, h->bins.tl);
     h->nbins_alloc = h->nbins;
     h->max_chunks = pmk::deposit_max_chunks(n, h->bins.tl, h->cap, h->bins.chunk);
-    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : 8;
+    h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : kResortIntervalDefault;
     if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
     {
         char* bp = static_cast<char*>(p);

@@ -81,7 +81,7 @@ typedef struct pm_config {
     int32_t mass_shift;    /* fixed-point bits (rho = q * 2^-shift); 0 = from the masses */
     int32_t device;        /* HIP device ordinal                                       */
     int32_t resort_interval; /* steps between re-binning the particles by lattice tile;
-                                0 = default (8).  Affects speed only, never results.    */
+                                0 = default (7).  Affects speed only, never results.    */
     int32_t reserved[6];
 } pm_config;
 

@@ -83,7 +83,7 @@ def test_headline_full_size():
                             / np.sum(fx64 ** 2 + fy64 ** 2)))
         say(f"field vs FP64: potential {e_phi:.2e}, force {e_f:.2e}", t0)
         assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD
-        s.step(30)                                 # crosses fused re-sort cycles (interval 8)
+        s.step(30)                                 # crosses fused re-sort cycles (interval 7)
         d = s.diagnostics()
         assert s.info().sorts >= 3
     check_invariants(d0, d, npart, arrs)

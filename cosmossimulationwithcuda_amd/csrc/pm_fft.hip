@@ -30,8 +30,9 @@
 // FFT: in-place Stockham on LDS, every thread of the block working across all
 // transforms of the block; stages of radix 4 or 8 (plan choice) after one
 // leading radix-2/4 stage for the leftover factor; twiddles exp(-2 pi i m/N)
-// from a quarter-wave table built in FP64 and staged in LDS; LDS rows are
-// padded against bank conflicts.
+// from a quarter-wave table built in FP64 and staged in LDS (stages whose
+// twiddle indices are far apart read per-stage copies of the same values);
+// LDS rows are padded against bank conflicts.
 //
 // Every pass reads its arrays from a FieldDesc; blockIdx.y selects the
 // instance, so one launch serves one simulation or a batch of simulations of
@@ -47,11 +48,11 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_r2c(const Fie
                                                      const float2* __restrict__ tw) {
     constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded(RP * N)];
-    __shared__ float2 twl[N / 4];
+    __shared__ float2 twl[tw_lds_entries(N, RADIX)];
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
-    load_twiddles<N, BT>(twl, tw);
+    load_twiddles<N, RADIX, BT>(twl, tw);
     for (int e = t; e < RP * N; e += BT) {
         const int f = e / N, i = e - f * N;
         const size_t ra = (size_t)(j0 + 2 * f) * N + i;
@@ -87,12 +88,12 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv(const Fi
                                                       const float2* __restrict__ tw) {
     constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded((FORCE ? 2 : 1) * CB * N)];
-    __shared__ float2 twl[N / 4];
+    __shared__ float2 twl[tw_lds_entries(N, RADIX)];
     const GField d = gfield(descs[blockIdx.y]);
     constexpr int NY = N / 2;                 // the Nyquist column's kx
     const int t = threadIdx.x;
     const int k0 = blockIdx.x * CB;
-    load_twiddles<N, BT>(twl, tw);
+    load_twiddles<N, RADIX, BT>(twl, tw);
     for (int e = t; e < CB * N; e += BT) {
         const int f = e / N, ky = e - f * N, k = k0 + f;
         buf[pad(e)] = k == 0 ? make_float2(d.H[ky].x, d.H[(size_t)NY * N + ky].x) : f2(d.H[(size_t)k * N + ky]);
@@ -159,14 +160,14 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_c2r(const Fie
     // transform batch (one read of the spectra, half the stage barriers).
     // Measured faster up to N = 1024; above, the larger LDS footprint halves
     // the blocks in flight and costs more than it saves.
-    constexpr bool MERGE = FORCE && N <= 1024 && padded(3 * RP * N) * 8 + 2 * N <= 160 * 1024;
+    constexpr bool MERGE = FORCE && N <= 1024 && padded(3 * RP * N) * 8 + 8 * tw_lds_entries(N, RADIX) <= 160 * 1024;
     __shared__ float2 buf[padded((MERGE ? 3 : (FORCE ? 2 : 1)) * RP * N)];
-    __shared__ float2 twl[N / 4];
+    __shared__ float2 twl[tw_lds_entries(N, RADIX)];
     const GField d = gfield(descs[blockIdx.y]);
     constexpr int NK = N / 2 + 1;
     const int t = threadIdx.x;
     const int j0 = row_group() * 2 * RP;
-    load_twiddles<N, BT>(twl, tw);
+    load_twiddles<N, RADIX, BT>(twl, tw);
     if constexpr (MERGE) {
         for (int e = t; e < NK * RP; e += BT) {
             const int k = e / RP, f = e - k * RP;
@@ -320,18 +321,18 @@ hipError_t c2r_t(const FieldDesc* d, int count, const float2* tw, bool force, hi
 
 // Row batches RP in {1,2,4} and column batches CB in {1,2,4,8} are compiled
 // where the LDS footprint (2*RP*N*8 or 2*CB*N*8 bytes with the force pass,
-// plus the 2*N-byte quarter twiddle table) fits 160 KB; plan_valid() states
-// the same rule for the host (with the 1/32 LDS padding).
-template <int N, int RP>
-constexpr bool rp_ok() { return padded(2 * RP * N) * 8 + 2 * N <= 160 * 1024 && 2 * RP <= N; }
+// plus the twiddle tables, 8*tw_lds_entries(N, R) bytes) fits 160 KB;
+// plan_valid() states the same rule for the host (with the 1/32 LDS padding).
+template <int N, int RP, int R>
+constexpr bool rp_ok() { return padded(2 * RP * N) * 8 + 8 * tw_lds_entries(N, R) <= 160 * 1024 && 2 * RP <= N; }
 // (k_row_c2r merges its two transform batches when three rows per pair fit.)
-template <int N, int CB>
-constexpr bool cb_ok() { return padded(2 * CB * N) * 8 + 2 * N <= 160 * 1024 && CB <= N / 2; }
+template <int N, int CB, int R>
+constexpr bool cb_ok() { return padded(2 * CB * N) * 8 + 8 * tw_lds_entries(N, R) <= 160 * 1024 && CB <= N / 2; }
 
 template <int N>
 hipError_t r2c_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s) {
 #define PM_R2C(RPV, RV) \
-    if constexpr (rp_ok<N, RPV>()) if (p.rp == RPV && p.radix == RV) return r2c_t<N, RPV, RV>(d, count, tw, s);
+    if constexpr (rp_ok<N, RPV, RV>()) if (p.rp == RPV && p.radix == RV) return r2c_t<N, RPV, RV>(d, count, tw, s);
     PM_R2C(1, 4) PM_R2C(2, 4) PM_R2C(4, 4) PM_R2C(1, 8) PM_R2C(2, 8) PM_R2C(4, 8)
     if constexpr (N >= 256) if (p.rp == 1 && p.radix == 16) return r2c_t<N, 1, 16>(d, count, tw, s);
 #undef PM_R2C
@@ -341,7 +342,7 @@ hipError_t r2c_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, hip
 template <int N>
 hipError_t col_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, bool force, hipStream_t s) {
 #define PM_COL(CBV, RV) \
-    if constexpr (cb_ok<N, CBV>()) if (p.cb == CBV && p.radix == RV) return col_t<N, CBV, RV>(d, count, tw, force, s);
+    if constexpr (cb_ok<N, CBV, RV>()) if (p.cb == CBV && p.radix == RV) return col_t<N, CBV, RV>(d, count, tw, force, s);
     PM_COL(1, 4) PM_COL(2, 4) PM_COL(4, 4) PM_COL(8, 4) PM_COL(1, 8) PM_COL(2, 8) PM_COL(4, 8) PM_COL(8, 8)
     if constexpr (N >= 256) if (p.cb == 1 && p.radix == 16) return col_t<N, 1, 16>(d, count, tw, force, s);
 #undef PM_COL
@@ -351,7 +352,7 @@ hipError_t col_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, boo
 template <int N>
 hipError_t c2r_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, bool force, hipStream_t s) {
 #define PM_C2R(RPV, RV) \
-    if constexpr (rp_ok<N, RPV>()) if (p.rp == RPV && p.radix == RV) return c2r_t<N, RPV, RV>(d, count, tw, force, s);
+    if constexpr (rp_ok<N, RPV, RV>()) if (p.rp == RPV && p.radix == RV) return c2r_t<N, RPV, RV>(d, count, tw, force, s);
     PM_C2R(1, 4) PM_C2R(2, 4) PM_C2R(4, 4) PM_C2R(1, 8) PM_C2R(2, 8) PM_C2R(4, 8)
     if constexpr (N >= 256) if (p.rp == 1 && p.radix == 16) return c2r_t<N, 1, 16>(d, count, tw, force, s);
 #undef PM_C2R
@@ -392,7 +393,8 @@ bool plan_valid(int n, FftPlan p) {
     if (p.radix != 4 && p.radix != 8) return false;
     if (p.rp != 1 && p.rp != 2 && p.rp != 4) return false;
     if (p.cb != 1 && p.cb != 2 && p.cb != 4 && p.cb != 8) return false;
-    const long lds_r = padded(2 * p.rp * n) * 8L + 2L * n, lds_c = padded(2 * p.cb * n) * 8L + 2L * n;
+    const long tw = 8L * tw_lds_entries(n, p.radix);
+    const long lds_r = padded(2 * p.rp * n) * 8L + tw, lds_c = padded(2 * p.cb * n) * 8L + tw;
     return lds_r <= 160 * 1024 && lds_c <= 160 * 1024 && 2 * p.rp <= n && p.cb <= n / 2;
 }
 

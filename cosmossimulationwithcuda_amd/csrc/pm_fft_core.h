@@ -48,6 +48,33 @@ __device__ __forceinline__ GField gfield(const FieldDesc& d) {
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
 
+// Per-stage twiddle tables.  A radix-R Stockham stage with sub-transform
+// length NS multiplies by w^(r*k*STEP), w = exp(-2 pi i/N), k = b mod NS,
+// STEP = N/(R*NS).  Read from the quarter table, lanes with different k hit
+// indices r*STEP apart, which for STEP >= 32 fall in one LDS bank (4- to
+// 16-way conflicts at N = 2048).  Stages with STEP > 1 read their own table
+// T[(r-1)*NS + k] instead (consecutive k across lanes: conflict-free), laid
+// out after the quarter table and filled from it, so the values are the same
+// FP32 numbers.  Tables above 1024 entries are left out (LDS budget at 8192).
+// Measured on the field solve: 4096^2 326 -> 308 us, 2048^2 83 -> 82 us; at
+// 256 and 512 the extra table loads cost more than the conflicts (+3-5%), so
+// the tables start at N = 1024.
+constexpr int first_radix_ns(int n, int r) {
+    return ilog2c(n) % ilog2c(r) ? 1 << (ilog2c(n) % ilog2c(r)) : 1;
+}
+constexpr bool stage_table(int n, int r, int ns) {
+    return n >= 1024 && r != 16 && ns > 1 && n / (r * ns) > 1 && (r - 1) * ns <= 1024;
+}
+constexpr int stage_table_off(int n, int r, int ns) {
+    int off = n / 4;
+    if (r == 16) return off;
+    for (int s = first_radix_ns(n, r); s < ns; s *= r)
+        if (stage_table(n, r, s)) off += (r - 1) * s;
+    return off;
+}
+// float2 entries of the LDS twiddle array of an n-point, radix-r transform
+constexpr int tw_lds_entries(int n, int r) { return stage_table_off(n, r, n); }
+
 // The reference's 4th-order stencil (cuh:174-229) on the FP32 potential:
 // ((-P[+2] + 8 P[+1]) - 8 P[-1]) + P[-2]; neighbours outside [0, N) take the
 // centre value; the force is this / 12 * G.
@@ -158,10 +185,26 @@ __device__ __forceinline__ int row_group() {
 template <int N, int RADIX>
 constexpr int fft_threads() { return N >= 4096 && RADIX != 16 ? 512 : kBlock; }
 
-// Loads the quarter twiddle table into LDS (callers synchronise after).
-template <int N, int BT = kBlock>
+template <int N, int R, int NS, int BT>
+__device__ __forceinline__ void load_stage_tables(float2* q, const float2* __restrict__ tw) {
+    if constexpr (NS < N) {
+        if constexpr (stage_table(N, R, NS)) {
+            constexpr int OFF = stage_table_off(N, R, NS), STEP = N / (R * NS);
+            for (int e = threadIdx.x; e < (R - 1) * NS; e += BT) {
+                const int r = e / NS + 1, k = e % NS;
+                q[OFF + e] = twiddle<N, false>(tw, r * k * STEP);
+            }
+        }
+        load_stage_tables<N, R, NS * R, BT>(q, tw);
+    }
+}
+
+// Loads the quarter twiddle table and the per-stage tables of a radix-R
+// transform into LDS (tw_lds_entries(N, R) entries; callers synchronise after).
+template <int N, int R, int BT = kBlock>
 __device__ __forceinline__ void load_twiddles(float2* q, const float2* __restrict__ tw) {
     for (int e = threadIdx.x; e < N / 4; e += BT) q[e] = tw[e];
+    if constexpr (R != 16) load_stage_tables<N, R, first_radix_ns(N, R), BT>(q, tw);
 }
 
 // One Stockham stage of radix R with sub-transform length NS over NFFT
@@ -188,7 +231,14 @@ __device__ __forceinline__ void stage(float2* buf, const float2* tw) {
                 for (int r = 0; r < R; ++r) v[p][r] = buf[pad(f * N + b + r * Q)];
             }
             const int k = b & (NS - 1);
-            if (NS > 1) {
+            if constexpr (stage_table(N, R, NS)) {
+                const float2* st = tw + stage_table_off(N, R, NS) + k;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = st[(r - 1) * NS];
+                    v[p][r] = cmul(v[p][r], make_float2(w.x, INV ? -w.y : w.y));
+                }
+            } else if (NS > 1) {
                 const int e = k * STEP;
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], twiddle<N, INV>(tw, r * e));

@@ -19,7 +19,7 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_r2c_it(const 
                                                         const float2* __restrict__ tw) {
     constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded(N)];
-    __shared__ float2 twl[N / 4];
+    __shared__ float2 twl[tw_lds_entries(N, RADIX)];
     constexpr int PER = N / BT, NK = N / 2 + 1;
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
@@ -33,7 +33,7 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_r2c_it(const 
             rb[p] = r[N + p * BT];
         }
     };
-    load_twiddles<N, BT>(twl, tw);
+    load_twiddles<N, RADIX, BT>(twl, tw);
     fetch(g0);
     for (int it = 0; it < IT; ++it) {
         const int j0 = 2 * (g0 + it);
@@ -71,7 +71,7 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv_it(const
                                                          const float2* __restrict__ tw) {
     constexpr int BT = fft_threads<N, RADIX>();
     __shared__ float2 buf[padded((FORCE ? 2 : 1) * N)];
-    __shared__ float2 twl[N / 4];
+    __shared__ float2 twl[tw_lds_entries(N, RADIX)];
     constexpr int NY = N / 2, PER = N / BT;
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
@@ -86,7 +86,7 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv_it(const
             tv[p] = d.T[(size_t)k * N + ky];
         }
     };
-    load_twiddles<N, BT>(twl, tw);
+    load_twiddles<N, RADIX, BT>(twl, tw);
     fetch(kb);
     for (int it = 0; it < IT; ++it) {
         const int k = kb + it;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_c2r_it(const 
     constexpr int BT = fft_threads<N, RADIX>();
     constexpr int ROWS = FORCE ? 3 : 1, NK = N / 2 + 1, PK = (NK + BT - 1) / BT;
     __shared__ float2 buf[padded(ROWS * N)];
-    __shared__ float2 twl[N / 4];
+    __shared__ float2 twl[tw_lds_entries(N, RADIX)];
     const GField d = gfield(descs[blockIdx.y]);
     const int t = threadIdx.x;
     const int g0 = row_group() * IT;
@@ -178,7 +178,7 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_c2r_it(const 
             }
         }
     };
-    load_twiddles<N, BT>(twl, tw);
+    load_twiddles<N, RADIX, BT>(twl, tw);
     fetch(g0);
     for (int it = 0; it < IT; ++it) {
         const int j0 = 2 * (g0 + it);
@@ -276,7 +276,7 @@ hipError_t launch_c2r(const FieldDesc* d, int count, const float2* tw, bool forc
 // pass 0 = R2C, 1 = column, 2 = C2R
 template <int N, int RADIX, int IT>
 hipError_t launch_pass(int pass, const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
-    static_assert(padded(3 * N) * 8 + 2 * N <= 160 * 1024, "C2R batch must fit LDS");
+    static_assert(padded(3 * N) * 8 + 8 * tw_lds_entries(N, RADIX) <= 160 * 1024, "C2R batch must fit LDS");
     if (pass == 0) return launch_r2c<N, RADIX, IT>(d, count, tw, s);
     if (pass == 1) return launch_col<N, RADIX, IT>(d, count, tw, force, s);
     return launch_c2r<N, RADIX, IT>(d, count, tw, force, s);

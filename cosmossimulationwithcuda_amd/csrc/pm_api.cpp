@@ -973,7 +973,9 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         const int tl_default = pmk::tile_log_for(n);
         for (int tl : {5, 6}) {
             if (tl < tl_default) continue;
-            for (int chunk : {8192, 12288, 16384, 32768, 65536, 131072}) {
+            std::vector<int> chunks = {8192, 12288, 16384, 32768, 65536, 131072};
+            if (h->np < pmk::kSmallRunParticles) chunks.insert(chunks.begin(), {2048, 4096});
+            for (int chunk : chunks) {
                 for (int resort : {4, 8, 12}) {
                     h->bins.tl = tl;
                     h->nbins = pmk::bin_count_of(n, tl);

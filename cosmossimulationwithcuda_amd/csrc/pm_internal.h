@@ -20,10 +20,20 @@ constexpr int kDepositChunkDefault = 12288;          // particles per particle-k
 // Measured: 12288 is fastest at 1e8 particles (2048^2 and 4096^2); at 1e9
 // (2048^2, dense disk cores) larger chunks win (CIC step 18.1 ms at 12288,
 // 16.7 at 65536, 15.3 at 131072): fewer blocks flush their deposit windows
-// into the same dense cells.
+// into the same dense cells.  Below kSmallRunParticles a 12288 floor leaves
+// most CUs idle (1e6 particles: 82 blocks), so chunks there are about
+// np / 256 within [2048, 8192], a multiple of 2048: measured on exponential
+// disks of 1.5e5..1e7 particles at 256^2..2048^2, the step kernel took 0.55x
+// to 1.0x the time it takes with 12288 (4e5 particles at 1024^2: 27.5 ->
+// 15.7 us; 1e7: 0.96..0.98x).
 constexpr int kChunkTargetBlocks = 8192;
 constexpr int kDepositChunkMax = 131072;
+constexpr int64_t kSmallRunParticles = 30000000;
 inline int auto_chunk(int64_t np) {
+    if (np < kSmallRunParticles) {
+        const int64_t c = (np / 256 + 2047) / 2048 * 2048;
+        return (int)(c < 2048 ? 2048 : c > 8192 ? 8192 : c);
+    }
     const int64_t c = (np / kChunkTargetBlocks + 4095) / 4096 * 4096;
     return (int)(c < kDepositChunkDefault ? kDepositChunkDefault : c > kDepositChunkMax ? kDepositChunkMax : c);
 }

@@ -526,15 +526,17 @@ def test_fft_prefetch_plan_rules():
 
 
 def test_deposit_chunk_follows_particle_count():
-    n, npart = 256, 300000
+    n, npart = 256, 700000
     x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=3)
     with make_sim(n, npart) as s:
         assert s.get_param("bin_max_chunks") == 0           # no per-tile cap by default
         s.load(x, y, vx, vy, m)
-        assert s.get_param("deposit_chunk") == 12288        # the floor (below ~1e8 particles)
-        s.set_param("deposit_chunk", 8192)
+        assert s.get_param("deposit_chunk") == 4096         # ~np/256 below 3e7 particles, multiple of 2048
+        s.load(x[:100000], y[:100000], vx[:100000], vy[:100000], m[:100000])
+        assert s.get_param("deposit_chunk") == 2048         # the small-run floor
+        s.set_param("deposit_chunk", 12288)
         s.load(x, y, vx, vy, m)
-        assert s.get_param("deposit_chunk") == 8192         # set explicitly: kept across loads
+        assert s.get_param("deposit_chunk") == 12288        # set explicitly: kept across loads
 
 
 def test_autotune_keeps_state():

@@ -67,6 +67,18 @@ int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 // headline case against 0.931 at 8, 0.930 at 6 and 1.03 at 10 (the re-sort
 // step itself grows with the particles' drift since the last sort).
 constexpr int kResortIntervalDefault = 7;
+// Large dense CIC runs (at least 3e7 particles, more than 64 per cell on
+// average) re-sort every 2 steps with the separate sort: there the fused
+// re-sort step and the window spills of drifted particles (four contended
+// global atomics each) cost more than a sort.  Measured on two colliding
+// disks of 1e9 particles on 2048^2 over steps 5-30 and 30-55: CIC 16.3 and
+// 22.1 ms per step at the default, 14.4 and 14.0 at 2; NGP 10.0 and 10.9 at
+// the default, 14.3 and 13.8 at 2; 1e8 and 2e7 particles are fastest at the
+// default.  Chosen at each load unless an interval was set.
+int auto_resort_interval(int mode, int64_t np, int n) {
+    const bool dense = np > (int64_t)pmk::kPipelineMaxDensity * n * n;
+    return mode == PM_CIC && np >= pmk::kSmallRunParticles && dense ? 2 : kResortIntervalDefault;
+}
 
 pmk::FftPlan plan_for(const pm_config& c) {
     if (c.precision == PM_FFT_BF16X3) {
@@ -396,6 +408,7 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
     }
     h->np = n;
     h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
+    if (h->resort_auto) h->resort_interval = auto_resort_interval(h->cfg.mode, n, h->cfg.n_lattice);
     if (h->chunk_auto && pmk::auto_chunk(n) != h->bins.chunk) {
         h->bins.chunk = pmk::auto_chunk(n);
         h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
@@ -461,6 +474,7 @@ pm_status reconfigure(pm_sim* h, const pm_config* cfg, const float* T_src) {
     h->cfg = *cfg;
     h->cfg.n_particles = cap;
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : kResortIntervalDefault;
+    h->resort_auto = cfg->resort_interval <= 0;
     pm_status st = PM_OK;
     if (spectra && (st = build_spectra(h, T_src)) != PM_OK) return st;
     if (tables && (st = upload_tables(h)) != PM_OK) return st;
@@ -577,6 +591,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
     h->nbins_alloc = h->nbins;
     h->max_chunks = pmk::deposit_max_chunks(n, h->bins.tl, h->cap, h->bins.chunk);
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : kResortIntervalDefault;
+    h->resort_auto = cfg->resort_interval <= 0;
     if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
     {
         char* bp = static_cast<char*>(p);
@@ -821,6 +836,7 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
     if (k == "resort_interval") {
         if (value < 1 || value > (1 << 20)) return fail(h, PM_EINVAL, "resort_interval must be in [1, 2^20]");
         h->resort_interval = (int)value;
+        h->resort_auto = false;
         return PM_OK;
     }
     if (k == "fused_sort") {
@@ -1018,6 +1034,7 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         h->chunk_auto = false;
         h->max_chunks = pmk::deposit_max_chunks(n, best_tl, h->cap, best_chunk);
         h->resort_interval = best_resort;
+        h->resort_auto = false;
         // the bin tables must match the restored particle order and the chosen tiles
         if (st == PM_OK) st = sort_now(h, false);
         // potential and force as the last field solve left them
@@ -1195,7 +1212,8 @@ pm_status pm_checkpoint_restore(const void* src, int64_t bytes, int32_t device, 
         if (pmk::plan_valid(c.n_lattice, q)) h->plan = q;
     }
     if (!pmk::plan_valid(c.n_lattice, h->plan)) h->plan = plan_for(c);
-    h->resort_interval = hd.resort_interval > 0 ? hd.resort_interval : 8;
+    h->resort_interval = hd.resort_interval > 0 ? hd.resort_interval : kResortIntervalDefault;
+    h->resort_auto = false;                 // the interval in force when the state was saved
     h->resort_permille = hd.resort_permille;
     if (hd.deposit_chunk > 0) {
         h->bins.chunk = hd.deposit_chunk;

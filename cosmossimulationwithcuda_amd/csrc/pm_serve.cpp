@@ -300,7 +300,10 @@ bool admit_one(pm_server* s) {
     // A large simulation's round is captured as a graph when the round is
     // exactly one re-sort cycle: use the round length as the interval unless
     // the request sets one (re-sorting never changes results).
-    if (st == PM_OK && s->cfg.use_graphs && r->cfg.resort_interval <= 0) h->resort_interval = s->cfg.round_steps;
+    if (st == PM_OK && s->cfg.use_graphs && r->cfg.resort_interval <= 0) {
+        h->resort_interval = s->cfg.round_steps;
+        h->resort_auto = false;
+    }
     if (st == PM_OK) remember_spectra(s, h);
     const double tu = now_s();
     if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);

@@ -40,7 +40,8 @@ struct pm_sim {
     int nbins = 0;                         // in-bounds bins + parked bin
     int nbins_alloc = 0;                   // bins the bin arrays were sized for
     int max_chunks = 0;
-    int resort_interval = 8;
+    int resort_interval = 7;
+    bool resort_auto = true;               // interval chosen at load by auto_resort_interval
     int steps_since_sort = 0;
     // adaptive re-sort: the migrant counter is copied to pinned host memory
     // after a push and read without blocking at a later push; when more than

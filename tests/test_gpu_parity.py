@@ -539,6 +539,26 @@ def test_deposit_chunk_follows_particle_count():
         assert s.get_param("deposit_chunk") == 12288        # set explicitly: kept across loads
 
 
+def test_resort_interval_follows_density():
+    """Large dense CIC runs (>= 3e7 particles, > 64 per cell) re-sort every 2
+    steps with the separate sort, others every 7; an interval set explicitly
+    is kept; the schedule never changes results (bit-identical trajectories)."""
+    n, npart = 512, 30_000_000
+    x, y, vx, vy, m = ics.uniform_square(npart, 32, 480, seed=9)
+    outs = []
+    for mode, interval, want in ((pm.CIC, 0, 2), (pm.CIC, 7, 7), (pm.NGP, 0, 7)):
+        with make_sim(n, npart, mode, dt=0.05) as s:
+            if interval:
+                s.set_param("resort_interval", interval)
+            s.load(x, y, vx, vy, m)
+            assert s.get_param("resort_interval") == want
+            if mode == pm.CIC:
+                s.step(5)
+                outs.append(s.read())
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_autotune_keeps_state():
     """The autotuner leaves the simulation state untouched: after tuning the
     particle parameters, 5 more steps are bit-identical to an untuned run;

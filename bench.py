@@ -307,6 +307,14 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
            "requests": nreq, "steps_per_request": SERVING["steps"],
            "lattices": list(SERVING["lattices"]), "particles_loguniform": list(SERVING["particles"]),
            "budget_GB": SERVING["budget_bytes"] / 2 ** 30}
+    # One untimed pass over the first 32 requests in each server mode: the
+    # first server run of a process pays one-time costs (code-object loading,
+    # first device allocations, graph instantiation; ~0.7 s measured) that
+    # would otherwise fall on whichever mode is timed first.
+    for name, mode in (("plain", pm.SERVE_PLAIN), ("batched", pm.SERVE_BATCHED)):
+        if name in modes:
+            print(f"# serving warm-up {name}", file=sys.stderr, flush=True)
+            serving_run(pm, ics, reqs[:32], templates, mode, False)
     for tag, poisson in (("saturated", False), ("poisson", True)):
         if tag not in arrivals:
             continue

@@ -197,14 +197,16 @@ def serving_workload(nreq, seed):
             for a, b, c, d, e in zip(lat, npart, kind, arrive, seeds)]
 
 
-def serving_run(pm, ics, reqs, templates, mode, poisson):
-    """Submits the requests (at their arrival times, or all at once), waits for
-    every result, returns the server statistics."""
+def serving_run(pm, reqs, arrays, mode, poisson):
+    """Submits the requests (at their arrival times, or all at once) with their
+    pre-generated arrays, waits for every result, returns the server
+    statistics."""
     import queue
     import threading
     srv = pm.Server(pm.default_server_config(mode=mode, hbm_budget_bytes=SERVING["budget_bytes"]))
     q = queue.Queue()
     errors = []
+    submitted = []
 
     def submitter():
         try:
@@ -214,10 +216,11 @@ def serving_run(pm, ics, reqs, templates, mode, poisson):
                     d = t0 + r["t"] - time.perf_counter()
                     if d > 0:
                         time.sleep(d)
-                x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
+                x, y, vx, vy, m, g = arrays[len(submitted)]
                 cfg = pm.default_config(n_lattice=r["n_lattice"], n_particles=r["n"], mode=pm.CIC,
                                         dt=SERVING["dt"], gravity=g, green_origin=1.0, deconv_eps=1e-3)
                 q.put(srv.submit(cfg, x, y, vx, vy, m, SERVING["steps"]))
+                submitted.append(r)
         except Exception as e:          # reported by the waiting side
             errors.append(e)
         q.put(None)
@@ -253,13 +256,12 @@ def serving_run(pm, ics, reqs, templates, mode, poisson):
                                                      "upload_s")}}
 
 
-def serving_naive_loop(pm, ics, reqs, templates):
+def serving_naive_loop(pm, reqs, arrays):
     """The literal one-at-a-time loop a caller writes without the server: for
     each request create a simulation, load, step, read back, destroy."""
     wall = 0.0
     steps = psteps = 0
-    for r in reqs:
-        x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
+    for r, (x, y, vx, vy, m, g) in zip(reqs, arrays):
         cfg = pm.default_config(n_lattice=r["n_lattice"], n_particles=r["n"], mode=pm.CIC, dt=SERVING["dt"],
                                 gravity=g, green_origin=1.0, deconv_eps=1e-3)
         t0 = time.perf_counter()
@@ -271,7 +273,8 @@ def serving_naive_loop(pm, ics, reqs, templates):
         steps += SERVING["steps"]
         psteps += SERVING["steps"] * r["n"]
     return {"wall_s": round(wall, 3), "steps_per_s": steps / wall, "particle_steps_per_s": psteps / wall,
-            "note": "time of create + load + steps + read-back + destroy per request, summed; array generation excluded"}
+            "note": "time of create + load + steps + read-back + destroy per request, summed; arrays generated "
+                    "before timing, as for the server modes"}
 
 
 def serving_cpu_baseline(ics, reqs, templates, steps=5):
@@ -301,7 +304,11 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
     t = time.perf_counter()
     templates = [ics.disk_template(k, SERVING["template"], seed + i) for i, k in enumerate(("exponential", "uniform"))]
     reqs = serving_workload(nreq, seed)
-    print(f"# serving: {nreq} requests, {sum(r['n'] for r in reqs)} particles total, templates in "
+    # Every request's arrays are generated once, before any timed mode: all
+    # modes then time the same work (generating 6.9e8 particles takes ~4 s on
+    # one host core, as long as the batched server's whole run).
+    arrays = [ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"]) for r in reqs]
+    print(f"# serving: {nreq} requests, {sum(r['n'] for r in reqs)} particles total, arrays in "
           f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
     out = {"workload": f"ensemble_{nreq}req_poisson{SERVING['rate_per_s']:g}_{SERVING['steps']}steps_cic",
            "requests": nreq, "steps_per_request": SERVING["steps"],
@@ -314,7 +321,7 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
     for name, mode in (("plain", pm.SERVE_PLAIN), ("batched", pm.SERVE_BATCHED)):
         if name in modes:
             print(f"# serving warm-up {name}", file=sys.stderr, flush=True)
-            serving_run(pm, ics, reqs[:32], templates, mode, False)
+            serving_run(pm, reqs[:32], arrays[:32], mode, False)
     for tag, poisson in (("saturated", False), ("poisson", True)):
         if tag not in arrivals:
             continue
@@ -322,7 +329,7 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
             if name not in modes:
                 continue
             print(f"# serving {tag} {name}", file=sys.stderr, flush=True)
-            out[f"{tag}_{name}"] = serving_run(pm, ics, reqs, templates, mode, poisson)
+            out[f"{tag}_{name}"] = serving_run(pm, reqs, arrays, mode, poisson)
         if "plain" in modes and "batched" in modes:
             p, b = out[f"{tag}_plain"], out[f"{tag}_batched"]
             out[f"{tag}_speedup_steps_per_s"] = round(b["steps_per_s"] / p["steps_per_s"], 3)
@@ -330,7 +337,7 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
         out["cpu_baseline"] = serving_cpu_baseline(ics, reqs[:16], templates)
     if "saturated" in arrivals and "naive" in modes:
         print("# serving naive per-request loop", file=sys.stderr, flush=True)
-        out["naive_loop"] = serving_naive_loop(pm, ics, reqs, templates)
+        out["naive_loop"] = serving_naive_loop(pm, reqs, arrays)
         if "saturated_batched" in out:
             out["saturated_speedup_vs_naive_loop"] = round(
                 out["saturated_batched"]["steps_per_s"] / out["naive_loop"]["steps_per_s"], 3)

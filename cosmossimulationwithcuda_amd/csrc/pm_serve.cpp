@@ -5,17 +5,23 @@
 // No counterpart in the reference (its host side runs one simulation per GPU,
 // CosmosCuda.cuh:1302-1395).
 //
-// Two host threads:
+// Three host threads:
 //   io      admits queued requests (pooled or new simulation, upload, first
-//           deposit) and reads finished ones back, on its own stream;
+//           deposit), on its own stream;
+//   rb      reads finished requests back and returns their simulations to
+//           the pool, on its own stream;
 //   worker  runs rounds over the admitted requests: small simulations in
 //           batched launches on one stream, the others spread over a few
 //           lane streams (captured round graphs or eager steps).
-// Hand-offs go through `ready` (io -> worker) and `finished` (worker -> io),
-// so uploads and read-backs overlap the rounds.  Every simulation's
+// Hand-offs go through `ready` (io -> worker), `finished` (worker -> rb) and
+// the pool (rb -> io), so uploads, read-backs and rounds overlap.  (With
+// admission and read-back on one thread, that thread was busy for the whole
+// saturated 512-request ensemble: 2.9 s admitting + 1.8 s reading back of a
+// 4.8 s run.)  Every simulation's
 // arithmetic is the same as alone: results do not depend on the mode or on
 // the other requests.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -129,10 +135,15 @@ struct pm_server {
     int64_t submitted = 0, completed = 0, failed = 0;
     int64_t rounds = 0, batched_rounds = 0, graph_launches = 0, graph_captures = 0, pool_hits = 0;
     double t_admit = 0, t_launch = 0, t_readback = 0, t_wait = 0, t_create = 0, t_upload = 0;
-    // io thread only
+    // idle simulations: filled by the rb thread, taken or evicted by the io
+    // thread; admission holds pool_mu from its pool lookup to its budget
+    // decision, and rb clears budget_blocked only after returning simulations,
+    // so a "blocked" decision never misses one returned meanwhile
+    std::mutex pool_mu;
     std::vector<pm_sim*> pool;
+    std::atomic<bool> budget_blocked{false};
+    // io thread only
     std::map<SpectraKey, float*> spectra;            // transfer functions by physics parameters
-    bool budget_blocked = false;
     // worker only
     std::vector<Request*> active;
     std::string batch_err;
@@ -141,8 +152,8 @@ struct pm_server {
     void* hscratch = nullptr;
     BatchPlan plan;
     // streams (created once, default priority)
-    hipStream_t batch_stream = nullptr, io_stream = nullptr, lanes[kLanes] = {};
-    std::thread worker, io;
+    hipStream_t batch_stream = nullptr, io_stream = nullptr, rb_stream = nullptr, lanes[kLanes] = {};
+    std::thread worker, io, rb;
 };
 
 namespace {
@@ -229,6 +240,7 @@ bool admit_one(pm_server* s) {
         r = s->queue.front();
         in_use = s->bytes_in_use;
     }
+    std::unique_lock<std::mutex> pl(s->pool_mu);
     pm_sim* h = take_pooled(s, r);
     bool pooled = h != nullptr;
     if (!h) {
@@ -262,6 +274,7 @@ bool admit_one(pm_server* s) {
                 return false;
             }
         }
+        pl.unlock();
         pm_config c = r->cfg;
         c.n_particles = capacity_class(r->n);
         c.device = s->cfg.device;
@@ -283,6 +296,7 @@ bool admit_one(pm_server* s) {
         s->bytes_in_use += h->bytes;
         s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_use);
     }
+    if (pl.owns_lock()) pl.unlock();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->queue.pop_front();
@@ -337,7 +351,7 @@ void read_back(pm_server* s, Request* r) {
     if (r->failed) {
         finish_failed(s, r, r->msg.empty() ? h->err : r->msg);
     } else {
-        h->stream = s->io_stream;
+        h->stream = s->rb_stream;
         r->x.resize((size_t)r->n);
         r->y.resize((size_t)r->n);
         r->vx.resize((size_t)r->n);
@@ -346,7 +360,10 @@ void read_back(pm_server* s, Request* r) {
             finish_failed(s, r, h->err);
         } else {
             r->sim = nullptr;
-            s->pool.push_back(h);
+            {
+                std::lock_guard<std::mutex> pl(s->pool_mu);
+                s->pool.push_back(h);
+            }
             r->m.clear();
             r->m.shrink_to_fit();
             std::lock_guard<std::mutex> lk(s->mu);
@@ -367,35 +384,46 @@ void io_main(pm_server* s) {
     (void)hipSetDevice(s->cfg.device);
     const int64_t max_active = s->cfg.mode == PM_SERVE_PLAIN ? 1 : std::max(1, s->cfg.max_active);
     for (;;) {
-        std::vector<Request*> done;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_io.wait(lk, [&] {
-                return !s->finished.empty() ||
-                       (!s->queue.empty() && s->slots_used < max_active && !s->budget_blocked) ||
+                return (!s->queue.empty() && s->slots_used < max_active && !s->budget_blocked) ||
                        (s->stop && s->queue.empty() && s->slots_used == 0);
             });
-            if (s->finished.empty() && s->stop && s->queue.empty() && s->slots_used == 0) break;
-            done.swap(s->finished);
+            if (s->stop && s->queue.empty() && s->slots_used == 0) break;
         }
-        if (!done.empty()) {
-            const double t0 = now_s();
-            for (Request* r : done) read_back(s, r);
-            s->budget_blocked = false;
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->t_readback += now_s() - t0;
-        }
-        // admit until blocked, returning to read-backs as soon as any are waiting
         const double t0 = now_s();
         while (!s->budget_blocked && admit_one(s)) {
-            std::lock_guard<std::mutex> lk(s->mu);
-            if (!s->finished.empty()) break;
         }
         std::lock_guard<std::mutex> lk(s->mu);
         s->t_admit += now_s() - t0;
     }
     std::lock_guard<std::mutex> lk(s->mu);
     s->cv_work.notify_all();
+    s->cv_io.notify_all();
+}
+
+void rb_main(pm_server* s) {
+    (void)hipSetDevice(s->cfg.device);
+    for (;;) {
+        std::vector<Request*> done;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_io.wait(lk, [&] {
+                return !s->finished.empty() || (s->stop && s->queue.empty() && s->slots_used == 0);
+            });
+            if (s->finished.empty()) break;                 // stopping, nothing left
+            done.swap(s->finished);
+        }
+        const double t0 = now_s();
+        for (Request* r : done) read_back(s, r);
+        s->budget_blocked = false;                          // after the simulations went back to the pool
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->t_readback += now_s() - t0;
+        s->cv_io.notify_all();                              // slots and memory may admit more
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cv_io.notify_all();
 }
 
 // ---- worker thread: rounds ------------------------------------------------------
@@ -786,7 +814,7 @@ static void free_server(pm_server* s) {
     if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
     if (s->dscratch) (void)hipFree(s->dscratch);
     if (s->hscratch) (void)hipHostFree(s->hscratch);
-    for (hipStream_t* q : {&s->batch_stream, &s->io_stream})
+    for (hipStream_t* q : {&s->batch_stream, &s->io_stream, &s->rb_stream})
         if (*q) (void)hipStreamDestroy(*q);
     for (hipStream_t& q : s->lanes)
         if (q) (void)hipStreamDestroy(q);
@@ -804,7 +832,7 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     if (!s) return PM_ENOMEM;
     s->cfg = *cfg;
     bool ok = hipSetDevice(cfg->device) == hipSuccess;
-    for (hipStream_t* q : {&s->batch_stream, &s->io_stream})
+    for (hipStream_t* q : {&s->batch_stream, &s->io_stream, &s->rb_stream})
         ok = ok && hipStreamCreateWithFlags(q, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& q : s->lanes) ok = ok && hipStreamCreateWithFlags(&q, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
@@ -813,6 +841,7 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     }
     s->worker = std::thread(worker_main, s);
     s->io = std::thread(io_main, s);
+    s->rb = std::thread(rb_main, s);
     *out = s;
     return PM_OK;
 }
@@ -957,6 +986,7 @@ pm_status pm_server_destroy(pm_server_handle s) {
         s->cv_work.notify_all();
     }
     if (s->io.joinable()) s->io.join();
+    if (s->rb.joinable()) s->rb.join();
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->cfg.device);
     free_server(s);

@@ -387,9 +387,15 @@ pm_status stage_push_deposit(pm_sim* h, bool next_cleared) {
     return PM_OK;
 }
 
+// Fixed-point shift from sum|m| (eight interleaved partial sums, added in a
+// fixed order: deterministic, and ~4x faster than one dependent chain).
 int auto_shift(const float* m, int64_t n) {
-    double s = 0.0;
-    for (int64_t i = 0; i < n; ++i) s += std::fabs((double)m[i]);
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t i = 0;
+    for (; i + 8 <= n; i += 8)
+        for (int k = 0; k < 8; ++k) p[k] += std::fabs((double)m[i + k]);
+    for (; i < n; ++i) p[i & 7] += std::fabs((double)m[i]);
+    const double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     if (!(s > 0.0) || !std::isfinite(s)) return 40;
     // cell sums stay below sum|m| * 2^shift + 2n <= 2^61 + 2^35 < 2^63
     int sh = (int)std::floor(61.0 - std::log2(s));
@@ -413,12 +419,14 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
         h->bins.chunk = pmk::auto_chunk(n);
         h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
     }
-    PM_HIP(h, pmk::iota_ids(h->cur.id, n, h->stream));
     h->rho_valid = false;
     h->field_parity = -1;
-    pm_status st = sort_now(h, false);
+    // inv_scale follows the shift; uploaded before the sort kernels, so its
+    // blocking copy waits for the particle uploads only
+    pm_status st = upload_field_desc(h);
     if (st != PM_OK) return st;
-    return upload_field_desc(h);   // inv_scale follows the shift
+    PM_HIP(h, pmk::iota_ids(h->cur.id, n, h->stream));
+    return sort_now(h, false);
 }
 
 // Stencil symbol d(k) = G (8 sin t - sin 2t)/6, t = 2 pi k/N (exactly 0 at

@@ -72,11 +72,20 @@ struct Request {
     int gphase = 0;
     bool graphs_tried = false;
     int rounds_done = 0;
+    // recorded after the admission work (upload, sort, first deposit) on the
+    // simulation's own stream; the worker's streams wait on it before first use
+    hipEvent_t admitted = nullptr;
 };
 
 void destroy_graphs(Request* r) {
     for (auto& g : r->graph)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+// Device objects a request holds until it is read back or fails.
+void release_request(Request* r) {
+    destroy_graphs(r);
+    if (r->admitted) { (void)hipEventDestroy(r->admitted); r->admitted = nullptr; }
 }
 
 template <typename T>
@@ -174,8 +183,8 @@ void destroy_sim(pm_server* s, pm_sim* h) {
 }
 
 void finish_failed(pm_server* s, Request* r, std::string msg) {   // by value: msg may live in r->sim
+    release_request(r);
     if (r->sim) {
-        destroy_graphs(r);
         pm_sim* h = r->sim;
         r->sim = nullptr;
         destroy_sim(s, h);
@@ -304,7 +313,10 @@ bool admit_one(pm_server* s) {
         if (pooled) s->pool_hits += 1;
     }
     r->sim = h;
-    h->stream = s->io_stream;
+    // admission work runs on the simulation's own stream, so one admission
+    // never waits behind another's kernels; no host wait for it either (the
+    // worker's streams wait on `admitted`)
+    h->stream = h->own_stream;
     // rounds are replayed from graphs with a fixed re-sort cadence
     h->resort_permille = s->cfg.use_graphs ? 0 : 10;
     pm_config c = r->cfg;
@@ -322,7 +334,9 @@ bool admit_one(pm_server* s) {
     const double tu = now_s();
     if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);
     if (st == PM_OK) st = pmi::stage_deposit(h);
-    if (st == PM_OK) st = pm_sync(h);
+    if (st == PM_OK && !r->admitted && hipEventCreateWithFlags(&r->admitted, hipEventDisableTiming) != hipSuccess)
+        st = PM_EDEVICE;
+    if (st == PM_OK && hipEventRecord(r->admitted, h->stream) != hipSuccess) st = PM_EDEVICE;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->t_upload += now_s() - tu;
@@ -347,7 +361,7 @@ bool admit_one(pm_server* s) {
 // simulation to the pool.
 void read_back(pm_server* s, Request* r) {
     pm_sim* h = r->sim;
-    destroy_graphs(r);
+    release_request(r);
     if (r->failed) {
         finish_failed(s, r, r->msg.empty() ? h->err : r->msg);
     } else {
@@ -761,6 +775,7 @@ void run_round(pm_server* s) {
 
 void worker_main(pm_server* s) {
     (void)hipSetDevice(s->cfg.device);
+    std::vector<Request*> fresh;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
@@ -769,10 +784,17 @@ void worker_main(pm_server* s) {
                        (s->stop && s->queue.empty() && s->slots_used == 0);
             });
             if (s->ready.empty() && s->active.empty()) break;   // stopping, nothing left
-            s->active.insert(s->active.end(), s->ready.begin(), s->ready.end());
-            s->ready.clear();
+            fresh.swap(s->ready);
+            s->active.insert(s->active.end(), fresh.begin(), fresh.end());
             s->n_active = (int64_t)s->active.size();
         }
+        // every stream a round may use waits for the newcomers' admission work
+        for (Request* r : fresh) {
+            if (!r->admitted) continue;
+            (void)hipStreamWaitEvent(s->batch_stream, r->admitted, 0);
+            for (hipStream_t q : s->lanes) (void)hipStreamWaitEvent(q, r->admitted, 0);
+        }
+        fresh.clear();
         run_round(s);
     }
 }
@@ -807,8 +829,10 @@ int32_t pm_plan_admission(const int64_t* need_bytes, int32_t n_queued, int64_t f
 
 static void free_server(pm_server* s) {
     for (auto* list : {&s->active, &s->ready, &s->finished})
-        for (Request* r : *list)
-            if (r->sim) { destroy_graphs(r); pm_destroy(r->sim); r->sim = nullptr; }
+        for (Request* r : *list) {
+            release_request(r);
+            if (r->sim) { pm_destroy(r->sim); r->sim = nullptr; }
+        }
     for (pm_sim* h : s->pool) pm_destroy(h);
     for (auto& kv : s->spectra) (void)hipFree(kv.second);
     if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
@@ -873,6 +897,9 @@ pm_status pm_server_submit(pm_server_handle s, const pm_config* cfg, const float
     r->vx.assign(vx, vx + n);
     r->vy.assign(vy, vy + n);
     r->m.assign(m, m + n);
+    // the shift pm_load would pick, computed here on the caller's thread so
+    // that admission does not make a pass over the masses
+    if (r->cfg.mass_shift <= 0) r->cfg.mass_shift = pmi::auto_shift(m, n);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) {
         s->err = "server is stopping";

@@ -97,6 +97,8 @@ pm_status validate(const pm_config* c, std::string* why);
 int64_t align256(int64_t b);
 pm_status upload_field_desc(pm_sim* h);
 pm_status sort_now(pm_sim* h, bool counted);
+// Fixed-point shift pm_load picks for these masses (cfg.mass_shift 0).
+int auto_shift(const float* m, int64_t n);
 pm_status stage_deposit(pm_sim* h);
 pm_status stage_field(pm_sim* h, bool clear_next = false);
 pm_status stage_push(pm_sim* h);

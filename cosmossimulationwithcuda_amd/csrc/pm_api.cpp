@@ -132,8 +132,12 @@ void release(pm_sim* h) {
     h->allocs.clear();
     if (h->mig_event) (void)hipEventDestroy(h->mig_event);
     if (h->mig_host) (void)hipHostFree(h->mig_host);
+    if (h->tables_host) (void)hipHostFree(h->tables_host);
+    if (h->tables_event) (void)hipEventDestroy(h->tables_event);
+    h->tables_event = nullptr;
     h->mig_event = nullptr;
     h->mig_host = nullptr;
+    h->tables_host = nullptr;
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     h->own_stream = nullptr;
     h->stream = nullptr;
@@ -194,8 +198,7 @@ pm_status upload_field_desc(pm_sim* h) {
         d[2 + p] = d[p];
         d[2 + p].rho_zero = h->rho_buf[p ^ 1];
     }
-    PM_HIP(h, hipMemcpyAsync(h->fdesc, d, sizeof(d), hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipStreamSynchronize(h->stream));
+    PM_HIP(h, pmk::set_field_desc(h->fdesc, d, h->stream));   // no host wait
     return PM_OK;
 }
 
@@ -433,8 +436,16 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
 // k = 0 and k = N/2), and twiddles exp(-2 pi i k/N), both evaluated in FP64.
 pm_status upload_tables(pm_sim* h) {
     const int n = h->cfg.n_lattice;
-    std::vector<float2> tw(n);
-    std::vector<float> dk(n);
+    // staged in a pinned buffer of the handle, so the copies need no host
+    // wait; the buffer is rewritten only after the previous copies ran
+    if (!h->tables_host) {
+        PM_HIP(h, hipHostMalloc(&h->tables_host, 12 * (size_t)n, hipHostMallocDefault));
+        PM_HIP(h, hipEventCreateWithFlags(&h->tables_event, hipEventDisableTiming));
+    } else {
+        PM_HIP(h, hipEventSynchronize(h->tables_event));
+    }
+    float2* tw = static_cast<float2*>(h->tables_host);
+    float* dk = reinterpret_cast<float*>(tw + n);
     const double pi = 3.14159265358979323846;
     for (int k = 0; k < n; ++k) {
         const double a = 2.0 * pi * k / n;
@@ -443,9 +454,9 @@ pm_status upload_tables(pm_sim* h) {
                     ? 0.0f
                     : (float)((double)h->cfg.gravity * (8.0 * std::sin(a) - std::sin(2.0 * a)) / 6.0);
     }
-    PM_HIP(h, hipMemcpyAsync(h->tw, tw.data(), 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipMemcpyAsync(h->dk, dk.data(), 4 * (size_t)n, hipMemcpyHostToDevice, h->stream));
-    PM_HIP(h, hipStreamSynchronize(h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->tw, tw, 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipMemcpyAsync(h->dk, dk, 4 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    PM_HIP(h, hipEventRecord(h->tables_event, h->stream));
     return PM_OK;
 }
 
@@ -456,12 +467,14 @@ pm_status upload_tables(pm_sim* h) {
 pm_status build_spectra(pm_sim* h, const float* T_src) {
     const int n = h->cfg.n_lattice;
     hipError_t e;
-    if (T_src) {
+    if (T_src) {                              // a device copy: stream order suffices, no host wait
         e = hipMemcpyAsync(h->T, T_src, (size_t)4 * (n / 2 + 1) * n, hipMemcpyDeviceToDevice, h->stream);
-    } else {
-        double* work = reinterpret_cast<double*>(h->rho_buf[0]);
-        e = pmk::build_transfer(n, h->cfg.mode, h->cfg.green_origin, h->cfg.deconv_eps, work, h->T, h->stream);
+        h->rho_valid = false;
+        return e != hipSuccess ? hip_fail(h, e, "transfer copy") : PM_OK;
     }
+    double* work = reinterpret_cast<double*>(h->rho_buf[0]);
+    e = pmk::build_transfer(n, h->cfg.mode, h->cfg.green_origin, h->cfg.deconv_eps, work, h->T, h->stream);
+    // built spectra may be read on other streams next (a server caches them)
     hipError_t e2 = hipStreamSynchronize(h->stream);
     h->rho_valid = false;
     if (e != hipSuccess) return hip_fail(h, e, "build_transfer");
@@ -643,7 +656,10 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
     *h->mig_host = 0;
     if ((st = upload_tables(h)) != PM_OK) return bail(st);
     if ((st = build_spectra(h, T_src)) != PM_OK) return bail(st);
-    // deterministic initial lattices (force/potential read before any step are zero)
+    if ((st = upload_field_desc(h)) != PM_OK) return bail(st);
+    // deterministic initial lattices (force/potential read before any step are
+    // zero); the wait below also covers the tables, spectra and descriptors
+    // (a server hands the handle to another stream next)
     if (pmk::fill_zero(h->phi, (size_t)4 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->rho_buf[0], (size_t)8 * n * n, h->stream) != hipSuccess ||
@@ -651,7 +667,6 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
         pmk::fill_zero(h->bins.count, (size_t)L.bins, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(fail(h, PM_EDEVICE, "lattice clear failed"));
-    if ((st = upload_field_desc(h)) != PM_OK) return bail(st);
     *out = h;
     return PM_OK;
 }

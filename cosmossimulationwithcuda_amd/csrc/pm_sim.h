@@ -49,6 +49,8 @@ struct pm_sim {
     // step, the particles are re-sorted before the next push
     int resort_permille = 10;
     unsigned long long* mig_host = nullptr;
+    void* tables_host = nullptr;           // pinned staging of the twiddle / stencil tables
+    hipEvent_t tables_event = nullptr;     // recorded after their last upload
     hipEvent_t mig_event = nullptr;
     bool mig_inflight = false;
     unsigned long long mig_last = 0;

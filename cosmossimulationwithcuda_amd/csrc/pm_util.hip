@@ -93,7 +93,24 @@ __global__ __launch_bounds__(kBlock) void k_diag_lattice(const int64_t* __restri
     }
 }
 
+struct FieldDesc4 {
+    FieldDesc d[4];
+};
+
+__global__ void k_set_field_desc(FieldDesc* __restrict__ dst, FieldDesc4 v) {
+    if (threadIdx.x < 4) dst[threadIdx.x] = v.d[threadIdx.x];
+}
+
 }  // namespace
+
+// The descriptors travel as kernel arguments (copied at launch), so the
+// caller's array may go out of scope at once and nothing waits for the copy.
+hipError_t set_field_desc(FieldDesc* dst, const FieldDesc* src4, hipStream_t s) {
+    FieldDesc4 v;
+    for (int i = 0; i < 4; ++i) v.d[i] = src4[i];
+    hipLaunchKernelGGL(k_set_field_desc, dim3(1), dim3(64), 0, s, dst, v);
+    return hipGetLastError();
+}
 
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
     const size_t n16 = bytes / 16, ntail = bytes - n16 * 16;

@@ -418,8 +418,11 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
     h->np = n;
     h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
     if (h->resort_auto) h->resort_interval = auto_resort_interval(h->cfg.mode, n, h->cfg.n_lattice);
-    if (h->chunk_auto && pmk::auto_chunk(n) != h->bins.chunk) {
-        h->bins.chunk = pmk::auto_chunk(n);
+    const int nl = h->cfg.n_lattice;
+    const bool fused_dense = n > (int64_t)pmk::kPipelineMaxDensity * nl * nl && h->fused_sort &&
+                             h->resort_interval >= 3 && pmk::scatter_fits(nl, h->bins.tl);
+    if (h->chunk_auto && pmk::auto_chunk(n, nl, fused_dense) != h->bins.chunk) {
+        h->bins.chunk = pmk::auto_chunk(n, nl, fused_dense);
         h->max_chunks = pmk::deposit_max_chunks(h->cfg.n_lattice, h->bins.tl, h->cap, h->bins.chunk);
     }
     h->rho_valid = false;

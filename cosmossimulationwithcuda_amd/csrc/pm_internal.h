@@ -15,29 +15,30 @@ constexpr int kBlock = 256;   // threads per block for every streaming kernel (4
 constexpr int kHalo = 7;
 constexpr int kDepositBlock = 512;
 constexpr int kDepositChunkDefault = 12288;          // particles per particle-kernel block (tunable)
-// Automatic chunk.  64-cell tiles and dense runs with fused re-sorts: about
-// kChunkTargetBlocks blocks per step-kernel launch, within
-// [kDepositChunkDefault, kDepositChunkMax], a multiple of 4096 (12288 at 1e8
-// on 4096^2; 122880 at 1e9 NGP on 2048^2, where fewer blocks flush their
-// deposit windows into the same dense cells).  Below kSmallRunParticles a 12288 floor leaves
-// most CUs idle (1e6 particles: 82 blocks), so chunks there are about
-// np / 256 within [2048, 8192], a multiple of 2048: measured on exponential
-// disks of 1.5e5..1e7 particles at 256^2..2048^2, the step kernel took 0.55x
-// to 1.0x the time it takes with 12288 (4e5 particles at 1024^2: 27.5 ->
-// 15.7 us; 1e7: 0.96..0.98x).
-// From 3e7 particles on 32-cell tiles (N <= 2048) chunks are kChunkTiled32
-// (measured after the step kernel's pipelining and LDS-only barriers; ms per
-// step / step kernel ms): two colliding disks of 1e8 on 2048^2, CIC 0.925 /
-// 0.771 at 12288, 0.907 / 0.742 at 8192, 0.899 / 0.728 at 7680, 0.905 /
-// 0.728 at 6656, 0.975 / 0.791 at 4096; NGP 0.924 at 12288, 0.888 at 7680;
-// 1e9 CIC (separate re-sort every 2 steps) 14.22 at 122880, 13.95 at 32768,
-// 13.77 at 16384, 13.46 at 7680.  Dense runs that re-sort inside the step
-// kernel (1e9 NGP) keep the large chunks (11.13 ms per step at 122880, 12.15
-// at 32768: the fused re-sort step slows with more blocks), and so do 64-cell
-// tiles (4096^2 Plummer spheres, 1e8: 1.22 at 12288, 1.32 at 7680).
+// Automatic chunk, chosen at each load:
+//  * below kSmallRunParticles: about np / 256 within [2048, 8192], a multiple
+//    of 2048 (a 12288 floor left most CUs idle: 1e6 particles, 82 blocks);
+//    measured on exponential disks of 1.5e5..1e7 particles at 256^2..2048^2,
+//    the step kernel took 0.55x to 1.0x the time it takes with 12288 (4e5
+//    particles at 1024^2: 27.5 -> 15.7 us; 1e7: 0.96..0.98x);
+//  * from there on 32-cell tiles (N <= 2048): kChunkTiled32.  Two colliding
+//    disks of 1e8 on 2048^2 (ms per step / step kernel ms): CIC 0.925 / 0.771
+//    at 12288, 0.907 / 0.742 at 8192, 0.899 / 0.728 at 7680, 0.905 / 0.728 at
+//    6656, 0.975 / 0.791 at 4096; NGP 0.924 at 12288, 0.888 at 7680; 1e9 CIC
+//    (separate re-sort every 2 steps) 14.22 at 122880, 13.95 at 32768, 13.77
+//    at 16384, 13.46 at 7680;
+//  * on 64-cell tiles (N >= 4096, windows 4x larger): kChunkTiled64.  Plummer
+//    spheres of 1e8 on 4096^2: 1.224 ms per step at 12288, 1.205 at 16384,
+//    1.191 at 20480, 1.197 at 24576, 1.32 at 7680;
+//  * dense runs whose re-sorts are fused into the step kernel (1e9 NGP on
+//    2048^2): about kChunkTargetBlocks blocks per launch within
+//    [kDepositChunkDefault, kDepositChunkMax], a multiple of 4096 (11.13 ms
+//    per step at 122880, 12.15 at 32768: the fused re-sort step slows with
+//    more blocks flushing into the same dense cells).
 constexpr int kChunkTargetBlocks = 8192;
 constexpr int kDepositChunkMax = 131072;
 constexpr int kChunkTiled32 = 7680;
+constexpr int kChunkTiled64 = 20480;
 constexpr int64_t kSmallRunParticles = 30000000;
 // large_chunks: a dense run whose re-sorts are fused into the step kernel
 inline int auto_chunk(int64_t np, int n, bool large_chunks) {
@@ -45,7 +46,7 @@ inline int auto_chunk(int64_t np, int n, bool large_chunks) {
         const int64_t c = (np / 256 + 2047) / 2048 * 2048;
         return (int)(c < 2048 ? 2048 : c > 8192 ? 8192 : c);
     }
-    if (n <= 2048 && !large_chunks) return kChunkTiled32;
+    if (!large_chunks) return n <= 2048 ? kChunkTiled32 : kChunkTiled64;
     const int64_t c = (np / kChunkTargetBlocks + 4095) / 4096 * 4096;
     return (int)(c < kDepositChunkDefault ? kDepositChunkDefault : c > kDepositChunkMax ? kDepositChunkMax : c);
 }

@@ -188,9 +188,9 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * their tile's window; 0 = off, default 10), "deposit_chunk" (particles per
  * particle-kernel block, a multiple of 512; by default set at each load:
  * below 3e7 particles about n_particles / 256 within [2048, 8192]; above,
- * 7680 on lattices up to 2048, or about n_particles / 8192 within
- * [12288, 131072] on larger lattices and for dense runs (> 64 particles per
- * cell) whose re-sorts run inside the step kernel; setting it fixes it),
+ * 7680 on lattices up to 2048 and 20480 above, or about n_particles / 8192
+ * within [12288, 131072] for dense runs (> 64 particles per cell) whose
+ * re-sorts run inside the step kernel; setting it fixes it),
  * "profile_mask" (kernels timed while pm_profile is on, bit k = PM_K_* id k),
  * "bin_max_chunks" (a dense tile is split into at most this many equal
  * blocks, default 0 = no cap), "tile_log" (bin edge 2^5 or

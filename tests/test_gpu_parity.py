@@ -159,6 +159,27 @@ def test_deposit_hot_cell_and_empty():
 
 
 # ---------------------------------------------------------------- field --
+def test_reload_updates_field_scale():
+    """Two loads in a row, no wait between, masses 2^20 apart (different
+    fixed-point shifts): the deposit and the field use the second load's
+    scale (the field descriptors are written in stream order)."""
+    n, npart = 256, 65536
+    x, y, vx, vy, m = ics.uniform_square(npart, 64, 192, seed=21)
+    with make_sim(n, npart) as s:
+        s.load(x, y, vx, vy, m * np.float32(2.0 ** 20))
+        shift_big = s.info().mass_shift
+        s.load(x, y, vx, vy, m)
+        shift = s.info().mass_shift
+        s.stage(pm.STAGE_DEPOSIT)
+        s.stage(pm.STAGE_FIELD)
+        rho = s.read_lattice(pm.LATTICE_MASS)
+        phi = s.read_lattice(pm.LATTICE_POTENTIAL)
+    assert shift == shift_big + 20
+    assert np.array_equal(rho, orc.deposit(n, pm.CIC, shift, x, y, m))
+    phi64 = orc.potential(n, orc.transfer(n, pm.CIC, 1.0, 1e-3), shift, rho)
+    assert rel_l2(phi, phi64) <= REL_L2_FIELD
+
+
 @pytest.mark.parametrize("n,mode", [(256, pm.CIC), (256, pm.NGP), (1024, pm.CIC)])
 def test_field_vs_fp64(n, mode):
     npart = 65536 if n == 256 else 400000

@@ -43,8 +43,10 @@ namespace {
 using Clock = std::chrono::steady_clock;
 double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
 
-constexpr int kLanes = 3;   // streams for non-batched simulations (3: 4.57 s on the saturated ensemble,
-                             // 2: 4.82 s, 4: 4.67 s)
+// Streams for non-batched simulations.  Saturated 512-request ensemble, with
+// separate admission and read-back threads: 2 lanes 4.79-4.83 s, 3: 4.51,
+// 4: 4.45-4.46, 5: 4.74-4.76, 6: 4.56-4.57.
+constexpr int kLanes = 4;
 
 int64_t capacity_class(int64_t n) {
     int64_t c = 1024;

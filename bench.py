@@ -354,6 +354,8 @@ def main():
     ap.add_argument("--breakdown-steps", type=int, default=16,
                     help="steps of the profiled pass after the timed region that times the non-step kernels")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--window-steps", type=int, default=180,
+                    help="after the timed region: reload and time steps 20..20+this (0 = skip)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (hipMemcpy) as the calibration "
@@ -398,7 +400,7 @@ def main():
 
     # Timed region: HIP events only around the step kernels (the roofline's
     # kernel); timing every launch would add ~30 us of event records per step.
-    step_kernels = ("push_deposit", "push_deposit_sort", "push_deposit_count")
+    step_kernels = ("push_deposit", "push_deposit_sort", "push_deposit_count", "sort")
     sim.set_param("profile_mask", sum(1 << pm.KERNELS.index(k) for k in step_kernels))
     sim.profile(True)
     sim.profile_reset()
@@ -436,8 +438,22 @@ def main():
             traffic = json.load(open(tfile)).get(args.workload, {}).get(dom)
         except Exception:
             traffic = None
+    # Amortised over the re-sort cycle: every launch of the particle step
+    # (steady, bin-counting and re-sorting variants, plus any separate sort
+    # pass, whose bytes are not algorithmic) -> summed bytes / summed time.
+    am_bytes = am_ms = 0.0
+    for k in step_kernels:
+        ms, cnt = prof.get(k, (0.0, 0))
+        if cnt:
+            am_bytes += algorithmic_bytes(k, n, np_) * cnt
+            am_ms += ms
+    frac_am = round(am_bytes / (am_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if am_ms > 0 else None
     roofline = {"bound": "hbm", "achieved": kd["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(kd["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom}
+                "frac": round(kd["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, "
+                                  "committed; not measured in this run)" if traffic is not None else None,
+                "kernel": dom, "frac_amortised": frac_am,
+                "frac_amortised_kernels": [k for k in step_kernels if prof.get(k, (0, 0))[1]]}
 
     value = np_ * args.steps / el
     out = {
@@ -461,6 +477,20 @@ def main():
                         f"separate {args.breakdown_steps}-step profiled pass after it",
         "diagnostics": {k: diag[k] for k in ("mass", "kinetic", "potential", "nonfinite", "in_bounds")},
     }
+    # Step time over the configuration's own window (steps 20-200 after a
+    # fresh load): the disks collapse as they approach, so the rate may drift.
+    if args.window_steps > 0:
+        sim.load(*arrs)
+        sim.step(20)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(args.window_steps)
+        sim.sync()
+        w_el = time.perf_counter() - t0
+        out["window"] = {"steps": f"20-{20 + args.window_steps}", "ms_per_step": round(w_el / args.window_steps * 1e3, 4),
+                         "particle_steps_per_s": np_ * args.window_steps / w_el}
+        print(f"# window steps 20-{20 + args.window_steps}: {w_el / args.window_steps * 1e3:.4f} ms/step",
+              file=sys.stderr, flush=True)
     del sim
     if args.copy_probe:
         copy_probe(1 << 30)

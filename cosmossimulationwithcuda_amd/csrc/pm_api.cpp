@@ -1214,8 +1214,10 @@ pm_status pm_checkpoint_restore(const void* src, int64_t bytes, int32_t device, 
     std::memcpy(&hd, src, sizeof(hd));
     if (std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != 1 || hd.header_bytes != sizeof(CkptHeader))
         return fail(nullptr, PM_EINVAL, "not a checkpoint of this format");
+    // n comes from the file: bound it before any size arithmetic (20 * n must not overflow)
     const int64_t n = hd.n_particles;
-    if (n < 0 || bytes < (int64_t)sizeof(CkptHeader) + 20 * n) return fail(nullptr, PM_EINVAL, "checkpoint truncated");
+    if (n < 0 || n > (bytes - (int64_t)sizeof(CkptHeader)) / 20)
+        return fail(nullptr, PM_EINVAL, "checkpoint truncated");
     const auto* in = static_cast<const unsigned char*>(src);
     if (fnv1a(in + sizeof(CkptHeader), 20 * (size_t)n) != hd.checksum)
         return fail(nullptr, PM_EINVAL, "checkpoint checksum mismatch");

@@ -1,6 +1,6 @@
 """GPU parity at the full workload sizes: the headline case (2048^2 lattice,
 1e8 particles in two colliding exponential disks, CIC, dt 0.002) and the
-one-GPU scale (the same disks with 1e9 particles, NGP).
+one-GPU scale (the same disks with 1e9 particles, NGP and CIC).
 
 At these sizes the checks are the ones a CPU run of seconds can back:
   * the deposit of the loaded state and one fused step, bit for bit against the
@@ -103,5 +103,27 @@ def test_one_gpu_scale_ngp():
         check_fused_step(s, n, pm.NGP, dt, arrs, t0)
         s.step(12)                                 # adaptive re-sorts fire every few steps at this mass
         d = s.diagnostics()
+    check_invariants(d0, d, npart, arrs)
+    say("invariants after 13 steps", t0)
+
+
+def test_one_gpu_scale_cic():
+    """1e9 particles in CIC mode: this density takes its own path (a separate
+    counting sort every 2 steps, four contended LDS atomics per particle)."""
+    t0 = time.perf_counter()
+    n, npart, dt = 2048, 1_000_000_000, 0.002
+    arrs = ics.colliding_disks_parallel(npart, 3)
+    say("initial conditions", t0)
+    cfg = pm.default_config(n_lattice=n, n_particles=npart, mode=pm.CIC, dt=dt, gravity=1.0, green_origin=1.0,
+                            deconv_eps=1e-3)
+    with pm.Simulation(cfg) as s:
+        s.load(*arrs)
+        assert s.get_param("resort_interval") == 2      # dense CIC: separate sort every 2 steps
+        d0 = s.diagnostics()
+        sorts0 = s.info().sorts
+        check_fused_step(s, n, pm.CIC, dt, arrs, t0)
+        s.step(12)
+        d = s.diagnostics()
+        assert s.info().sorts - sorts0 >= 6               # the sort cadence was crossed
     check_invariants(d0, d, npart, arrs)
     say("invariants after 13 steps", t0)

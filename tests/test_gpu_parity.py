@@ -514,10 +514,15 @@ def test_fft_prefetch_plans_identical(n, radix, its, force_mode):
     for it in its[1:]:
         assert np.array_equal(out[it][0].view(np.uint32), out[1][0].view(np.uint32))
         assert np.array_equal(out[it][1].view(np.uint32), out[1][1].view(np.uint32))
-    if n <= 2048:
-        T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
-        last = out[its[-1]]
-        assert rel_l2(last[0], orc.potential(n, T, last[3], last[2])) <= REL_L2_FIELD
+    T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
+    last = out[its[-1]]
+    phi64 = orc.potential(n, T, last[3], last[2])
+    assert rel_l2(last[0], phi64) <= REL_L2_FIELD
+    if force_mode == pm.FORCE_SPECTRAL:
+        fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+        f = last[1]
+        e_f = float(np.sqrt(np.sum((f[..., 0] - fx64) ** 2 + (f[..., 1] - fy64) ** 2) / np.sum(fx64 ** 2 + fy64 ** 2)))
+        assert e_f <= REL_L2_FIELD
 
 
 def test_fft_prefetch_plan_rules():

@@ -121,3 +121,53 @@ def test_invalid_request_rejected_at_submit():
         with pytest.raises(pm.PMError) as e:
             srv.submit(bad, z, z, z, z, z, steps=1)
         assert e.value.status == pm.PM_EINVAL and "power of two" in str(e.value)
+
+
+# ---- configuration 5 at its own sizes ----------------------------------------------
+# The first 24 requests of the benchmark's 512-request ensemble (seed 3): lattices
+# 256 to 2048, 1.1e4 to 9.2e6 particles (2048^2 with 6.0e6; 256^2 with 9.2e6, >100 per
+# cell), exponential and uniform disks, CIC, 200 steps each, dt 1, the request's own
+# gravity; run through the batched server with captured graphs, all submitted at
+# once and as a stream whose later requests arrive while earlier ones run.
+C5_REQUESTS = 24
+
+
+@pytest.fixture(scope="module")
+def c5():
+    import bench
+    reqs = bench.serving_workload(512, 3)[:C5_REQUESTS]
+    assert any(r["n_lattice"] == 2048 and r["n"] >= 1_000_000 for r in reqs)
+    assert sum(r["n"] >= 1_000_000 for r in reqs) >= 8
+    templates = [ics.disk_template(k, bench.SERVING["template"], 3 + i) for i, k in enumerate(("exponential", "uniform"))]
+    cfgs, arrays = [], []
+    for r in reqs:
+        x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
+        cfgs.append(pm.default_config(n_lattice=r["n_lattice"], n_particles=r["n"], mode=pm.CIC,
+                                      dt=bench.SERVING["dt"], gravity=g, green_origin=1.0, deconv_eps=1e-3))
+        arrays.append((x, y, vx, vy, m))
+    steps = bench.SERVING["steps"]
+    want = [run_alone(c, a, steps) for c, a in zip(cfgs, arrays)]
+    return reqs, cfgs, arrays, steps, want
+
+
+@pytest.mark.parametrize("arrivals", ["saturated", "stream"])
+def test_config5_ensemble_matches_alone_bitwise(c5, arrivals):
+    import time
+    reqs, cfgs, arrays, steps, want = c5
+    print(f"config 5 subset: {len(reqs)} requests, {sum(r['n'] for r in reqs)} particles, {arrivals}", flush=True)
+    with pm.Server(mode=pm.SERVE_BATCHED, use_graphs=1) as srv:
+        ids = []
+        for c, a in zip(cfgs, arrays):
+            ids.append(srv.submit(c, *a, steps=steps))
+            if arrivals == "stream":
+                time.sleep(0.01)
+        got = [srv.wait(i, timeout_s=300) for i in ids]
+        stats = srv.stats()
+    for k, (g, w) in enumerate(zip(got, want)):
+        for a, b in zip(g, w):
+            assert np.array_equal(a, b, equal_nan=True), f"request {k} {reqs[k]} differs from its alone run"
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
+    assert stats["batched_launch_rounds"] > 0 and stats["graph_launches"] > 0
+    print(f"rounds {stats['rounds']}, batched {stats['batched_launch_rounds']}, graph launches "
+          f"{stats['graph_launches']}, step latency p50/p99 {stats['step_latency_p50_ms']:.3f}/"
+          f"{stats['step_latency_p99_ms']:.3f} ms", flush=True)

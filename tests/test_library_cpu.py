@@ -120,3 +120,20 @@ def test_checkpoint_restore_validates_before_device():
     assert b"not a checkpoint" in pm.lib().pm_last_error(None)
     assert pm.lib().pm_checkpoint_restore(junk, 16, 0, ctypes.byref(h)) == pm.PM_EINVAL
     assert pm.lib().pm_checkpoint_bytes(None) == -1
+
+
+def test_checkpoint_restore_rejects_oversized_count_before_arithmetic():
+    """A header whose particle count would overflow 20 * n is rejected as
+    truncated (the count is bounded by the buffer before any size arithmetic)."""
+    import struct
+    cfg = pm.default_config(n_lattice=256, n_particles=1000)
+    cfg_bytes = bytes(cfg)
+    hdr_size = 16 + len(cfg_bytes) + 8 * 4 + 8 * 3          # magic, sizes, pm_config, 8 int32, n, steps, checksum
+    for n in (1 << 62, (1 << 63) - 1, 1000):
+        hdr = (b"PMCKPT\x00\x01" + struct.pack("<II", hdr_size, 1) + cfg_bytes + struct.pack("<8i", 40, 1, 1, 4, 7, 10, 2048, 5)
+               + struct.pack("<qqQ", n, 0, 0))
+        assert len(hdr) == hdr_size
+        buf = (ctypes.c_ubyte * (hdr_size + 64)).from_buffer_copy(hdr + bytes(64))
+        h = ctypes.c_void_p()
+        assert pm.lib().pm_checkpoint_restore(buf, hdr_size + 64, 0, ctypes.byref(h)) == pm.PM_EINVAL
+        assert b"truncated" in pm.lib().pm_last_error(None)

@@ -237,6 +237,7 @@ hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
 // Writes four field descriptors to device memory, in stream order.
 hipError_t set_field_desc(FieldDesc* dst, const FieldDesc* src4, hipStream_t s);
 // Per-block FP64 partial sums: out[b*8 + {0..7}] = {mass, px, py, ke, max|v|, nonfinite, n_inb, 0}
+constexpr int kDiagBlocks = 1024;   // partial sums of the diagnostic reductions: at most this many blocks
 int diag_blocks(int64_t np);
 hipError_t diag_particles(int n, const float* x, const float* y, const float* vx, const float* vy,
                           const float* m, int64_t np, double* partial, hipStream_t s);

@@ -2,7 +2,10 @@
 // CosmosCuda.cuh:315-331) and FP64 diagnostic reductions (mass, momentum,
 // kinetic and potential energy, max |v|, non-finite count), which have no
 // counterpart in the reference.
-#include "pm_internal.h"
+#include <cmath>
+#include <mutex>
+
+#include "pm_sim.h"
 
 namespace pmk {
 namespace {
@@ -36,7 +39,6 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-constexpr int kDiagBlocks = 1024;
 
 __global__ __launch_bounds__(kBlock) void k_diag_particles(const float* __restrict__ x, const float* __restrict__ y,
                                                             const float* __restrict__ vx,
@@ -156,3 +158,93 @@ hipError_t diag_lattice(int n, const int64_t* rho, double inv_scale, const float
 }
 
 }  // namespace pmk
+
+// ---- diagnostics without allocation -------------------------------------------
+// pm_diagnostics_noalloc: the reductions of pm_diagnostics over caller-owned
+// device arrays, for callers that cannot allocate device memory.  The block
+// partial sums live in `scratch`, a fixed array of this code object (each
+// device holds its own copy until the process ends); host code reaches it
+// only through the per-device address hipGetSymbolAddress returns.
+namespace {
+
+constexpr int kPartOff = 0;                                  // particle partials: 8 doubles per block
+constexpr int kLatOff = 8 * pmk::kDiagBlocks;                // lattice partials: 1 double per block
+constexpr int kScratchDoubles = kLatOff + pmk::kDiagBlocks;  // 9216 doubles = 72 KiB
+static_assert(kScratchDoubles * sizeof(double) == PM_DIAG_NOALLOC_SCRATCH_BYTES, "scratch size");
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Final reduction in the order pm_diagnostics uses on the host: thread q < 7
+// walks column q of the particle partials block by block (q = 4 is a max),
+// thread 7 sums the lattice partials; the same inputs in the same order give
+// the same bits.
+__global__ void k_diag_finish(const double* __restrict__ part, int pb, const double* __restrict__ lat, int lb,
+                              double* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q < 7) {
+        double v = 0.0;
+        for (int b = 0; b < pb; ++b) v = q == 4 ? fmax(v, part[b * 8 + 4]) : v + part[b * 8 + q];
+        // out: mass, px, py, ke, pe, max|v|, nonfinite, in-bounds
+        out[q < 4 ? q : q + 1] = v;
+    } else if (q == 7) {
+        double pe = 0.0;
+        for (int b = 0; b < lb; ++b) pe += lat[b];
+        out[4] = 0.5 * pe;
+    }
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+hipError_t scratch_on(int device, double** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" pm_status pm_diagnostics_noalloc(int32_t device, void* stream, int32_t n_lattice, const float* x,
+                                            const float* y, const float* vx, const float* vy, const float* m,
+                                            int64_t np, const int64_t* rho, int32_t mass_shift, const float* phi,
+                                            double* out) {
+    if (!out) return pmi::fail(nullptr, PM_EINVAL, "null output buffer");
+    if (np < 0) return pmi::fail(nullptr, PM_EINVAL, "np < 0");
+    if (np > 0 && (!x || !y || !vx || !vy || !m)) return pmi::fail(nullptr, PM_EINVAL, "null particle array");
+    if (n_lattice < 64 || n_lattice > 8192 || (n_lattice & (n_lattice - 1)))
+        return pmi::fail(nullptr, PM_EINVAL, "n_lattice must be a power of two in [64, 8192]");
+    if ((rho == nullptr) != (phi == nullptr)) return pmi::fail(nullptr, PM_EINVAL, "rho and phi: both or neither");
+    if (rho && (mass_shift < 0 || mass_shift > 62)) return pmi::fail(nullptr, PM_EINVAL, "mass_shift outside [0, 62]");
+    // the partial-sum counts are capped by construction; the scratch holds the caps
+    const int pb = pmk::diag_blocks(np > 0 ? np : 1), lb = rho ? pmk::diag_lattice_blocks(n_lattice) : 0;
+    if (pb > pmk::kDiagBlocks || lb > pmk::kDiagBlocks)
+        return pmi::fail(nullptr, PM_EINVAL, "reduction exceeds the fixed scratch array");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= kMaxDevices)
+        return pmi::fail(nullptr, PM_ENODEV, "no such device");
+    int prev = -1;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    double* sp = nullptr;
+    if (e == hipSuccess) e = scratch_on(device, &sp);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (e == hipSuccess && np > 0) e = pmk::diag_particles(n_lattice, x, y, vx, vy, m, np, sp + kPartOff, s);
+    if (e == hipSuccess && rho)
+        e = pmk::diag_lattice(n_lattice, rho, std::ldexp(1.0, -mass_shift), phi, sp + kLatOff, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_diag_finish, dim3(1), dim3(64), 0, s, sp + kPartOff, np > 0 ? pb : 0, sp + kLatOff, lb,
+                           out);
+        e = hipGetLastError();
+    }
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return e == hipSuccess ? PM_OK : pmi::hip_fail(nullptr, e, "pm_diagnostics_noalloc");
+}

@@ -107,6 +107,8 @@ def lib():
         L.pm_profile_read.argtypes = [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64)]
         L.pm_profile_reset.argtypes = [_H]
+        L.pm_diagnostics_noalloc.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32] + \
+            [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         L.pm_stream.argtypes = [_H]
         L.pm_stream.restype = ctypes.c_void_p
         L.pm_last_error.argtypes = [_H]

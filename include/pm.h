@@ -227,6 +227,24 @@ pm_status pm_profile(pm_handle h, int32_t enable);
 pm_status pm_profile_read(pm_handle h, int32_t kernel, double* total_ms, int64_t* launches);
 pm_status pm_profile_reset(pm_handle h);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  pm_diagnostics' FP64 reductions over caller-owned
+ * DEVICE arrays, written to a caller-owned DEVICE buffer: x, y, vx, vy, m
+ * (np floats each), optionally the int64 mass lattice rho (fixed point
+ * 2^-mass_shift) and the float potential phi of an n_lattice^2 lattice (both
+ * NULL: out[4] = 0), out = 8 doubles in pm_diagnostics' order.  Allocates
+ * nothing: the block partial sums use a fixed device array of the library,
+ * PM_DIAG_NOALLOC_SCRATCH_BYTES per device, held until the process ends.
+ * Asynchronous on `stream` (a hipStream_t; NULL = the null stream) of
+ * `device`; returns PM_EINVAL for arguments the fixed array cannot hold.
+ * Every call on one device shares that array: calls that overlap on one
+ * device race.  Same inputs in the same order give pm_diagnostics' bits. */
+#define PM_DIAG_NOALLOC_SCRATCH_BYTES 73728
+pm_status pm_diagnostics_noalloc(int32_t device, void* stream, int32_t n_lattice, const float* x, const float* y,
+                                 const float* vx, const float* vy, const float* m, int64_t np, const int64_t* rho,
+                                 int32_t mass_shift, const float* phi, double* out);
+
 /* The handle's hipStream_t, for callers that order their own work after it. */
 void* pm_stream(pm_handle h);
 

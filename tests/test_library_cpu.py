@@ -137,3 +137,14 @@ def test_checkpoint_restore_rejects_oversized_count_before_arithmetic():
         h = ctypes.c_void_p()
         assert pm.lib().pm_checkpoint_restore(buf, hdr_size + 64, 0, ctypes.byref(h)) == pm.PM_EINVAL
         assert b"truncated" in pm.lib().pm_last_error(None)
+
+
+def test_diagnostics_noalloc_validates_before_device():
+    L = pm.lib()
+    out = ctypes.c_void_p(1)               # never dereferenced: every call below is rejected first
+    assert L.pm_diagnostics_noalloc(0, None, 100, None, None, None, None, None, 0, None, 0, None, out) == pm.PM_EINVAL
+    assert b"power of two" in L.pm_last_error(None)
+    assert L.pm_diagnostics_noalloc(0, None, 256, None, None, None, None, None, 5, None, 0, None, out) == pm.PM_EINVAL
+    assert L.pm_diagnostics_noalloc(0, None, 256, None, None, None, None, None, 0, None, 0, None, None) == pm.PM_EINVAL
+    assert L.pm_diagnostics_noalloc(0, None, 256, None, None, None, None, None, 0, ctypes.c_void_p(8), 0, None,
+                                    out) == pm.PM_EINVAL

@@ -915,7 +915,7 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         if (k != "fft_it" && p.it != 1 && !pmk::plan_valid(h->cfg.n_lattice, p)) p.it = 1;
         if (!pmk::plan_valid(h->cfg.n_lattice, p)) return fail(h, PM_EINVAL, "FFT plan not available for this lattice");
         if ((p.radix == 16) != (h->cfg.precision == PM_FFT_BF16X3))
-            return fail(h, PM_EINVAL, "fft_radix 16 is the PM_FFT_BF16X3 precision; 4 or 8 are FP32");
+            return fail(h, PM_EINVAL, "fft_radix 16 is the PM_FFT_BF16X3 precision; 4, 8 or 64 are FP32");
         h->plan = p;
         return PM_OK;
     }
@@ -978,7 +978,7 @@ pm_status pm_autotune(pm_handle h, int32_t flags, int32_t reps) {
         if (!h->rho_valid && (st = stage_deposit(h)) != PM_OK) { h->profiling = prof; return st; }
         pmk::FftPlan best = h->plan;
         float best_ms = 1e30f;
-        for (int radix : {4, 8})
+        for (int radix : {4, 8, pmk::kWaveRadix})
             for (int rp : {1, 2, 4})
                 for (int cb : {1, 2, 4, 8})
                 for (int it : {1, 2, 4}) {

@@ -388,6 +388,7 @@ hipError_t stencil_n(const FieldDesc* d, int count, hipStream_t s) {
     }
 
 bool plan_valid(int n, FftPlan p) {
+    if (p.radix == kWaveRadix) return p.rp == 1 && p.cb == 1 && p.it == 1 && wave_plan_compiled(n);
     if (p.it != 1) return p.rp == 1 && p.cb == 1 && pipe_plan_compiled(n, p.radix, p.it);
     if (p.radix == 16) return p.rp == 1 && p.cb == 1 && n >= 256 && n <= 8192;
     if (p.radix != 4 && p.radix != 8) return false;
@@ -415,18 +416,21 @@ FftPlan default_plan(int n) {
 }
 
 hipError_t row_r2c(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, hipStream_t s) {
+    if (p.radix == kWaveRadix) return wave_pass(0, n, d, count, tw, false, s);
     if (p.it != 1) return row_r2c_pipe(n, d, count, tw, p, s);
     PM_DISPATCH_N(n, (r2c_n<NN>(d, count, tw, p, s)));
 }
 
 hipError_t col_conv(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
                     hipStream_t s) {
+    if (p.radix == kWaveRadix) return wave_pass(1, n, d, count, tw, with_force, s);
     if (p.it != 1) return col_conv_pipe(n, d, count, tw, p, with_force, s);
     PM_DISPATCH_N(n, (col_n<NN>(d, count, tw, p, with_force, s)));
 }
 
 hipError_t row_c2r(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
                    hipStream_t s) {
+    if (p.radix == kWaveRadix) return wave_pass(2, n, d, count, tw, with_force, s);
     if (p.it != 1) return row_c2r_pipe(n, d, count, tw, p, with_force, s);
     PM_DISPATCH_N(n, (c2r_n<NN>(d, count, tw, p, with_force, s)));
 }

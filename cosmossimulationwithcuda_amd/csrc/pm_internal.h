@@ -171,6 +171,11 @@ struct FieldDesc {
 struct FftPlan {
     int rp = 2, cb = 1, radix = 4, it = 1;
 };
+// radix kWaveRadix: the wave-local passes (pm_fft_wave.hip), one 64-lane
+// wavefront per row pair / column, rp = cb = it = 1, N = 256 .. 2048
+constexpr int kWaveRadix = 64;
+bool wave_plan_compiled(int n);
+hipError_t wave_pass(int pass, int n, const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s);
 bool plan_valid(int n, FftPlan p);
 bool pipe_plan_compiled(int n, int radix, int it);
 // The prefetching C2R pass (it > 1) applies the reference's clamped stencil to

@@ -661,3 +661,46 @@ def test_fused_resort_matches_separate_sort(mode, wrap):
     assert np.array_equal(ra, rb)
     for u, v in zip(a, b):
         assert np.array_equal(u, v, equal_nan=True)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048])
+@pytest.mark.parametrize("force_mode", [pm.FORCE_SPECTRAL, pm.FORCE_STENCIL])
+def test_wave_plan_field_vs_fp64(n, force_mode):
+    """The wave-local FFT passes (fft_radix 64: one wavefront per row pair /
+    column): potential (and spectral force) within 1e-5 of FP64 on the same
+    lattice, edge bands (or every cell in stencil mode) bit-exact against the
+    reference stencil on the GPU potential, and the same field as the block
+    passes to FP32 rounding."""
+    npart = 200000
+    x, y, vx, vy, m = ics.uniform_square(npart, n // 4, 3 * n // 4, seed=n + 5)
+    out = {}
+    for radix in (64, None):
+        with make_sim(n, npart, force_mode=force_mode) as s:
+            if radix:
+                s.set_param("fft_radix", radix)
+                assert s.get_param("fft_radix") == radix and s.get_param("fft_it") == 1
+            s.load(x, y, vx, vy, m)
+            s.stage(pm.STAGE_DEPOSIT)
+            s.stage(pm.STAGE_FIELD)
+            out[radix] = (s.read_lattice(pm.LATTICE_POTENTIAL), s.read_lattice(pm.LATTICE_FORCE),
+                          s.read_lattice(pm.LATTICE_MASS), s.info().mass_shift)
+    phi, force, rho, shift = out[64]
+    assert np.array_equal(rho, out[None][2])
+    T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
+    phi64 = orc.potential(n, T, shift, rho)
+    e_phi = rel_l2(phi, phi64)
+    f32 = orc.gradient32(n, 1.0, phi)
+    if force_mode == pm.FORCE_SPECTRAL:
+        fx64, fy64 = orc.gradient64(n, 1.0, phi64)
+        e_f = float(np.sqrt(np.sum((force[..., 0] - fx64) ** 2 + (force[..., 1] - fy64) ** 2)
+                            / np.sum(fx64 ** 2 + fy64 ** 2)))
+        for sl in (np.s_[:, :2], np.s_[:, n - 2:]):
+            assert np.array_equal(force[sl][..., 0], f32[sl][..., 0])
+        for sl in (np.s_[:2, :], np.s_[n - 2:, :]):
+            assert np.array_equal(force[sl][..., 1], f32[sl][..., 1])
+    else:
+        e_f = 0.0
+        assert np.array_equal(force, f32)
+    d_block = rel_l2(phi, out[None][0].astype(np.float64))
+    print(f"N={n} wave plan: potential {e_phi:.2e} force {e_f:.2e}; vs block plan {d_block:.2e}")
+    assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD and d_block <= 1e-6

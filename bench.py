@@ -180,7 +180,9 @@ def precision_report(seed):
 
 # Serving ensemble: independent requests arriving as a Poisson stream.
 SERVING = {"requests": 512, "rate_per_s": 40.0, "lattices": (256, 512, 1024, 2048), "particles": (1e4, 1e7),
-           "steps": 200, "budget_bytes": 200 << 30, "dt": 1.0, "template": 10_000_000}
+           "steps": 200, "budget_bytes": 200 << 30, "dt": 1.0, "template": 10_000_000,
+           # pinned host staging for the submitted arrays: the 512 requests hold 13.7 GB
+           "pinned_mib": 20480}
 
 
 def serving_workload(nreq, seed):
@@ -197,13 +199,16 @@ def serving_workload(nreq, seed):
             for a, b, c, d, e in zip(lat, npart, kind, arrive, seeds)]
 
 
-def serving_run(pm, reqs, arrays, mode, poisson):
+def serving_run(pm, reqs, arrays, mode, poisson, use_graphs=1):
     """Submits the requests (at their arrival times, or all at once) with their
     pre-generated arrays, waits for every result, returns the server
     statistics."""
     import queue
     import threading
-    srv = pm.Server(pm.default_server_config(mode=mode, hbm_budget_bytes=SERVING["budget_bytes"]))
+    t = time.perf_counter()
+    srv = pm.Server(pm.default_server_config(mode=mode, hbm_budget_bytes=SERVING["budget_bytes"],
+                                             pinned_staging_mib=SERVING["pinned_mib"], use_graphs=use_graphs))
+    t_create = time.perf_counter() - t
     q = queue.Queue()
     errors = []
     submitted = []
@@ -250,6 +255,7 @@ def serving_run(pm, reqs, arrays, mode, poisson):
             "request_latency_p50_ms": round(st["request_latency_p50_ms"], 2),
             "request_latency_p99_ms": round(st["request_latency_p99_ms"], 2),
             "peak_GB": round(st["peak_bytes"] / 1e9, 2), "rounds": st["rounds"],
+            "server_create_s": round(t_create, 3), "pinned_staging_GiB": SERVING["pinned_mib"] / 1024,
             "graph_launches": st["graph_launches"], "batched_rounds": st["batched_launch_rounds"],
             "pool_hits": st["pool_hits"], "graph_captures": st["graph_captures"], "failed": st["failed"],
             "worker_s": {k: round(st[k], 3) for k in ("admit_s", "launch_s", "readback_s", "wait_s", "create_s",
@@ -299,7 +305,7 @@ def serving_cpu_baseline(ics, reqs, templates, steps=5):
                       f"{el:.1f}s; oracle/pm_oracle.c, host {cpu_model()}"}
 
 
-def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson")):
+def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson"), use_graphs=1):
     from cosmossimulationwithcuda_amd import ics, pm
     t = time.perf_counter()
     templates = [ics.disk_template(k, SERVING["template"], seed + i) for i, k in enumerate(("exponential", "uniform"))]
@@ -321,7 +327,7 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
     for name, mode in (("plain", pm.SERVE_PLAIN), ("batched", pm.SERVE_BATCHED)):
         if name in modes:
             print(f"# serving warm-up {name}", file=sys.stderr, flush=True)
-            serving_run(pm, reqs[:32], arrays[:32], mode, False)
+            serving_run(pm, reqs[:32], arrays[:32], mode, False, use_graphs)
     for tag, poisson in (("saturated", False), ("poisson", True)):
         if tag not in arrivals:
             continue
@@ -329,7 +335,7 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
             if name not in modes:
                 continue
             print(f"# serving {tag} {name}", file=sys.stderr, flush=True)
-            out[f"{tag}_{name}"] = serving_run(pm, reqs, arrays, mode, poisson)
+            out[f"{tag}_{name}"] = serving_run(pm, reqs, arrays, mode, poisson, use_graphs)
         if "plain" in modes and "batched" in modes:
             p, b = out[f"{tag}_plain"], out[f"{tag}_batched"]
             out[f"{tag}_speedup_steps_per_s"] = round(b["steps_per_s"] / p["steps_per_s"], 3)
@@ -367,6 +373,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
     ap.add_argument("--serving-modes", default="plain,batched,naive,cpu")
     ap.add_argument("--serving-arrivals", default="saturated,poisson")
+    ap.add_argument("--serving-graphs", type=int, default=1, help="replay rounds from captured HIP graphs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -381,7 +388,7 @@ def main():
         return
     if args.serving == "only":
         print(json.dumps({"serving": serving_bench(args.serving_requests, args.seed, args.serving_modes.split(","),
-                                   args.serving_arrivals.split(","))}), flush=True)
+                                   args.serving_arrivals.split(","), args.serving_graphs)}), flush=True)
         return
 
     n, np_, mode, dt, gen = WORKLOADS[args.workload]
@@ -508,7 +515,7 @@ def main():
         out["cpu_baseline"] = None
     if args.serving == "on":
         out["serving"] = serving_bench(args.serving_requests, args.seed, args.serving_modes.split(","),
-                                   args.serving_arrivals.split(","))
+                                       args.serving_arrivals.split(","), args.serving_graphs)
     print(json.dumps(out), flush=True)
 
 

@@ -600,15 +600,18 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
         return bail(fail(h, PM_EDEVICE, "hipStreamCreate failed"));
     }
     h->stream = work_stream ? work_stream : h->own_stream;
+    // every array in one device allocation (one hipMalloc instead of two dozen;
+    // the Layout sizes are multiples of 256 bytes)
+    void* slab = nullptr;
+    if ((st = dalloc(h, &slab, L.total)) != PM_OK) return bail(st);
+    char* next = static_cast<char*>(slab);
+    auto carve = [&](int64_t bytes) { char* q = next; next += bytes; return static_cast<void*>(q); };
     void* p;
     float** parts[10] = {&h->cur.x, &h->cur.y, &h->cur.vx, &h->cur.vy, &h->cur.m,
                          &h->alt.x, &h->alt.y, &h->alt.vx, &h->alt.vy, &h->alt.m};
-    for (auto q : parts) {
-        if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st);
-        *q = static_cast<float*>(p);
-    }
-    if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->cur.id = static_cast<uint32_t*>(p);
-    if ((st = dalloc(h, &p, L.particles)) != PM_OK) return bail(st); h->alt.id = static_cast<uint32_t*>(p);
+    for (auto q : parts) *q = static_cast<float*>(carve(L.particles));
+    h->cur.id = static_cast<uint32_t*>(carve(L.particles));
+    h->alt.id = static_cast<uint32_t*>(carve(L.particles));
     h->bins.chunk = pmk::kDepositChunkDefault;
     h->bins.tl = pmk::tile_log_for(n);
     h->nbins = pmk::bin_count_of(n, h->bins.tl);
@@ -616,7 +619,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
     h->max_chunks = pmk::deposit_max_chunks(n, h->bins.tl, h->cap, h->bins.chunk);
     h->resort_interval = cfg->resort_interval > 0 ? cfg->resort_interval : kResortIntervalDefault;
     h->resort_auto = cfg->resort_interval <= 0;
-    if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
+    p = carve(L.bins);
     {
         char* bp = static_cast<char*>(p);
         const int64_t ib = align256(4 * (int64_t)(h->nbins + 1));
@@ -626,7 +629,7 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
         h->bins.start = reinterpret_cast<int64_t*>(bp + 3 * ib);
         h->migrants = reinterpret_cast<unsigned long long*>(bp + 3 * ib + align256(8 * (int64_t)(h->nbins + 2)));
     }
-    if ((st = dalloc(h, &p, L.bins)) != PM_OK) return bail(st);
+    p = carve(L.bins);
     {
         char* bp = static_cast<char*>(p);
         const int64_t ib = align256(4 * (int64_t)(h->nbins + 1));
@@ -637,20 +640,19 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
         if (pmk::fill_zero(p, (size_t)L.bins, h->stream) != hipSuccess) return bail(fail(h, PM_EDEVICE, "clear"));
     }
     // both mass lattices in one block: 2 * 8 N^2 bytes, also the FP64 setup work area
-    if ((st = dalloc(h, &p, 2 * L.rho)) != PM_OK) return bail(st);
-    h->rho_buf[0] = static_cast<int64_t*>(p);
+    h->rho_buf[0] = static_cast<int64_t*>(carve(2 * L.rho));
     h->rho_buf[1] = h->rho_buf[0] + L.rho / 8;
-    if ((st = dalloc(h, &p, L.fdesc)) != PM_OK) return bail(st);
-    h->fdesc = static_cast<pmk::FieldDesc*>(p);
+    h->fdesc = static_cast<pmk::FieldDesc*>(carve(L.fdesc));
     h->plan = plan_for(*cfg);
-    if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->H = static_cast<float2*>(p);
-    if ((st = dalloc(h, &p, L.H)) != PM_OK) return bail(st); h->Hfy = static_cast<float2*>(p);
-    if ((st = dalloc(h, &p, L.T)) != PM_OK) return bail(st); h->T = static_cast<float*>(p);
-    if ((st = dalloc(h, &p, L.dk)) != PM_OK) return bail(st); h->dk = static_cast<float*>(p);
-    if ((st = dalloc(h, &p, L.tw)) != PM_OK) return bail(st); h->tw = static_cast<float2*>(p);
-    if ((st = dalloc(h, &p, L.phi)) != PM_OK) return bail(st); h->phi = static_cast<float*>(p);
-    if ((st = dalloc(h, &p, L.force)) != PM_OK) return bail(st); h->force = static_cast<float2*>(p);
-    if ((st = dalloc(h, &p, L.diag)) != PM_OK) return bail(st); h->diag = static_cast<double*>(p);
+    h->H = static_cast<float2*>(carve(L.H));
+    h->Hfy = static_cast<float2*>(carve(L.H));
+    h->T = static_cast<float*>(carve(L.T));
+    h->dk = static_cast<float*>(carve(L.dk));
+    h->tw = static_cast<float2*>(carve(L.tw));
+    h->phi = static_cast<float*>(carve(L.phi));
+    h->force = static_cast<float2*>(carve(L.force));
+    h->diag = static_cast<double*>(carve(L.diag));
+    if (next - static_cast<char*>(slab) != L.total) return bail(fail(h, PM_EDEVICE, "layout mismatch"));
 
     if (hipHostMalloc(reinterpret_cast<void**>(&h->mig_host), sizeof(unsigned long long), hipHostMallocDefault) !=
             hipSuccess ||
@@ -661,15 +663,19 @@ pm_status create(const pm_config* cfg, pm_sim** out, const float* T_src, hipStre
     if ((st = build_spectra(h, T_src)) != PM_OK) return bail(st);
     if ((st = upload_field_desc(h)) != PM_OK) return bail(st);
     // deterministic initial lattices (force/potential read before any step are
-    // zero); the wait below also covers the tables, spectra and descriptors
-    // (a server hands the handle to another stream next)
+    // zero).  When the set-up ran on a caller's stream (a server's creation
+    // stream), the handle's own stream is ordered after it by an event: no
+    // host wait, so creating a simulation does not queue behind other work.
     if (pmk::fill_zero(h->phi, (size_t)4 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->force, (size_t)8 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->rho_buf[0], (size_t)8 * n * n, h->stream) != hipSuccess ||
         pmk::fill_zero(h->H, (size_t)8 * nk * n, h->stream) != hipSuccess ||
-        pmk::fill_zero(h->bins.count, (size_t)L.bins, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
+        pmk::fill_zero(h->bins.count, (size_t)L.bins, h->stream) != hipSuccess)
         return bail(fail(h, PM_EDEVICE, "lattice clear failed"));
+    if (h->stream != h->own_stream &&
+        (hipEventRecord(h->tables_event, h->stream) != hipSuccess ||
+         hipStreamWaitEvent(h->own_stream, h->tables_event, 0) != hipSuccess))
+        return bail(fail(h, PM_EDEVICE, "set-up ordering failed"));
     *out = h;
     return PM_OK;
 }
@@ -746,6 +752,15 @@ pm_status pm_sync(pm_handle h) {
 pm_status pm_read(pm_handle h, float* x, float* y, float* vx, float* vy) {
     if (!h) return fail(nullptr, PM_EINVAL, "null handle");
     PM_HIP(h, hipSetDevice(h->cfg.device));
+    const pm_status st = pmi::read_async(h, x, y, vx, vy);
+    if (st != PM_OK) return st;
+    PM_HIP(h, hipStreamSynchronize(h->stream));
+    return PM_OK;
+}
+
+}  // extern "C"
+
+pm_status pmi::read_async(pm_sim* h, float* x, float* y, float* vx, float* vy) {
     const size_t b = (size_t)h->np * 4;
     if (b) {
         // slots hold particles in bin order; scatter them back to load order in the scratch set
@@ -757,9 +772,10 @@ pm_status pm_read(pm_handle h, float* x, float* y, float* vx, float* vy) {
         if (vx) PM_HIP(h, hipMemcpyAsync(vx, h->alt.vx, b, hipMemcpyDeviceToHost, h->stream));
         if (vy) PM_HIP(h, hipMemcpyAsync(vy, h->alt.vy, b, hipMemcpyDeviceToHost, h->stream));
     }
-    PM_HIP(h, hipStreamSynchronize(h->stream));
     return PM_OK;
 }
+
+extern "C" {
 
 pm_status pm_read_verlet(pm_handle h, float* x, float* y, float* x_old, float* y_old) {
     if (!h) return fail(nullptr, PM_EINVAL, "null handle");

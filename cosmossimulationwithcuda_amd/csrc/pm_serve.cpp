@@ -54,12 +54,78 @@ int64_t capacity_class(int64_t n) {
     return c;
 }
 
+// Pinned host memory for request arrays, carved from one hipHostMalloc block
+// taken when the server is created: uploads and read-backs from it are
+// asynchronous DMA with no host wait.  (From pageable memory the
+// saturated 512-request ensemble spent 3.5 s of its 4.5 s admitting, most of
+// it in the 2,560 pageable copies, each waiting for its own completion while
+// rounds ran.)  Blocks come in size classes of four per octave from 64 KiB
+// and return to per-class free lists; a request that finds no room keeps its
+// arrays in pageable memory.
+class PinnedArena {
+  public:
+    ~PinnedArena() {
+        if (base_) (void)hipHostFree(base_);
+    }
+    bool init(size_t bytes) {
+        if (bytes == 0) return true;
+        void* p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return false;
+        base_ = static_cast<char*>(p);
+        bytes_ = bytes;
+        return true;
+    }
+    // a block of at least `need` bytes and its class, or nullptr
+    char* take(size_t need, size_t* cls) {
+        const size_t c = class_of(need);
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.find(c);
+        if (it != free_.end() && !it->second.empty()) {
+            char* q = it->second.back();
+            it->second.pop_back();
+            *cls = c;
+            return q;
+        }
+        if (!base_ || top_ + c > bytes_) return nullptr;
+        char* q = base_ + top_;
+        top_ += c;
+        *cls = c;
+        return q;
+    }
+    void give(char* q, size_t cls) {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_[cls].push_back(q);
+    }
+
+  private:
+    static size_t class_of(size_t need) {
+        size_t c = (size_t)1 << 16;
+        while (c < need) c <<= 1;
+        for (size_t m : {5, 6, 7})
+            if (c / 8 * m >= need) return c / 8 * m;
+        return c;
+    }
+    std::mutex mu_;
+    char* base_ = nullptr;
+    size_t bytes_ = 0, top_ = 0;
+    std::map<size_t, std::vector<char*>> free_;
+};
+
 struct Request {
     int64_t id = 0;
     pm_config cfg{};
     int64_t n = 0;
     int32_t steps = 0;
-    std::vector<float> x, y, vx, vy, m;
+    // x, y, vx, vy, m: n floats each, in one pinned block when the arena had
+    // room, else in `pageable`; read back into the first four
+    float* arr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<float> pageable;
+    PinnedArena* arena = nullptr;
+    char* pinned = nullptr;
+    size_t pinned_cls = 0;
+    ~Request() {
+        if (pinned) arena->give(pinned, pinned_cls);
+    }
     double t_submit = 0, t_admit = 0, t_done = 0;
     int32_t done_steps = 0;
     int32_t state = PM_REQ_QUEUED;
@@ -74,6 +140,14 @@ struct Request {
     int gphase = 0;
     bool graphs_tried = false;
     int rounds_done = 0;
+    // round pipelining: steps enqueued so far, the lane stream the request is
+    // bound to once it runs outside the batched launches (-1 = none yet), and
+    // whether its last round ran in the batched launches (a lane round after
+    // one waits for the batch stream)
+    int32_t launched_steps = 0;
+    int lane = -1;
+    double lane_cost = 0;
+    bool last_batched = false;
     // recorded after the admission work (upload, sort, first deposit) on the
     // simulation's own stream; the worker's streams wait on it before first use
     hipEvent_t admitted = nullptr;
@@ -124,6 +198,22 @@ struct BatchPlan {
 
 using SpectraKey = std::tuple<int, int, double, double>;   // lattice, mode, G(0), eps
 
+// A launched round: what it advances, the events recorded at its end on each
+// stream it used (index 0 = batch stream, 1 + l = lane l), and batched graphs
+// it replaced (destroyed once it has completed).
+struct InFlight {
+    double t_launch = 0;
+    struct Advance {
+        Request* r;
+        int steps;
+        int stream;                      // 0 = batch stream, 1 + l = lane l
+    };
+    std::vector<Advance> advance;
+    std::vector<Request*> completing;    // reach their step count (or failed) with this round
+    hipEvent_t ev[1 + kLanes] = {};
+    std::vector<hipGraphExec_t> retired;
+};
+
 }  // namespace
 
 struct pm_server {
@@ -131,6 +221,7 @@ struct pm_server {
     std::mutex mu;
     std::condition_variable cv_io, cv_work, cv_done;
     // guarded by mu
+    PinnedArena arena;                               // declared before reqs: outlives them
     std::deque<Request*> queue;                      // submitted, not admitted
     std::map<int64_t, std::unique_ptr<Request>> reqs;
     std::vector<Request*> ready;                     // admitted (io -> worker)
@@ -160,8 +251,15 @@ struct pm_server {
     std::string batch_err;
     void* dscratch = nullptr;
     size_t dscratch_bytes = 0;
-    void* hscratch = nullptr;
+    void* hscratch[2] = {nullptr, nullptr};          // descriptor staging, alternating between plan builds
+    hipEvent_t hscratch_done[2] = {nullptr, nullptr};
+    int hscratch_next = 0;
     BatchPlan plan;
+    std::deque<InFlight> inflight;                   // launched rounds, oldest first (at most two)
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t batch_done = nullptr;                 // re-recorded after every batched launch
+    double lane_load[kLanes] = {};
+    double t_last_complete = 0;
     // streams (created once, default priority)
     hipStream_t batch_stream = nullptr, io_stream = nullptr, rb_stream = nullptr, lanes[kLanes] = {};
     std::thread worker, io, rb;
@@ -334,7 +432,7 @@ bool admit_one(pm_server* s) {
     }
     if (st == PM_OK) remember_spectra(s, h);
     const double tu = now_s();
-    if (st == PM_OK) st = pm_load(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data(), r->m.data(), r->n);
+    if (st == PM_OK) st = pm_load(h, r->arr[0], r->arr[1], r->arr[2], r->arr[3], r->arr[4], r->n);
     if (st == PM_OK) st = pmi::stage_deposit(h);
     if (st == PM_OK && !r->admitted && hipEventCreateWithFlags(&r->admitted, hipEventDisableTiming) != hipSuccess)
         st = PM_EDEVICE;
@@ -359,40 +457,45 @@ bool admit_one(pm_server* s) {
     return true;
 }
 
-// Reads a finished request back (on the io stream) and returns its
-// simulation to the pool.
-void read_back(pm_server* s, Request* r) {
-    pm_sim* h = r->sim;
-    release_request(r);
-    if (r->failed) {
-        finish_failed(s, r, r->msg.empty() ? h->err : r->msg);
-    } else {
-        h->stream = s->rb_stream;
-        r->x.resize((size_t)r->n);
-        r->y.resize((size_t)r->n);
-        r->vx.resize((size_t)r->n);
-        r->vy.resize((size_t)r->n);
-        if (pm_read(h, r->x.data(), r->y.data(), r->vx.data(), r->vy.data()) != PM_OK) {
-            finish_failed(s, r, h->err);
-        } else {
-            r->sim = nullptr;
-            {
-                std::lock_guard<std::mutex> pl(s->pool_mu);
-                s->pool.push_back(h);
-            }
-            r->m.clear();
-            r->m.shrink_to_fit();
-            std::lock_guard<std::mutex> lk(s->mu);
-            r->state = PM_REQ_DONE;
-            r->t_done = now_s();
-            s->completed += 1;
-            s->t_last_done = r->t_done;
-            s->req_lat_ms.push_back((float)((r->t_done - r->t_submit) * 1e3));
-            s->cv_done.notify_all();
+// Reads finished requests back on the read-back stream: every copy is
+// enqueued first and waited for once, then each simulation returns to the
+// pool.
+void read_back(pm_server* s, const std::vector<Request*>& done) {
+    std::vector<Request*> ok;
+    for (Request* r : done) {
+        release_request(r);
+        if (r->failed) {
+            pm_sim* h = r->sim;
+            finish_failed(s, r, r->msg.empty() ? h->err : r->msg);
+            continue;
         }
+        r->sim->stream = s->rb_stream;
+        if (pmi::read_async(r->sim, r->arr[0], r->arr[1], r->arr[2], r->arr[3]) != PM_OK)
+            finish_failed(s, r, r->sim->err);
+        else
+            ok.push_back(r);
+    }
+    if (!ok.empty() && hipStreamSynchronize(s->rb_stream) != hipSuccess) {
+        for (Request* r : ok) finish_failed(s, r, "read-back failed");
+        ok.clear();
+    }
+    for (Request* r : ok) {
+        pm_sim* h = r->sim;
+        r->sim = nullptr;
+        {
+            std::lock_guard<std::mutex> pl(s->pool_mu);
+            s->pool.push_back(h);
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->state = PM_REQ_DONE;
+        r->t_done = now_s();
+        s->completed += 1;
+        s->t_last_done = r->t_done;
+        s->req_lat_ms.push_back((float)((r->t_done - r->t_submit) * 1e3));
+        s->cv_done.notify_all();
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    s->slots_used -= 1;
+    s->slots_used -= (int64_t)done.size();
     s->cv_work.notify_all();
 }
 
@@ -432,7 +535,7 @@ void rb_main(pm_server* s) {
             done.swap(s->finished);
         }
         const double t0 = now_s();
-        for (Request* r : done) read_back(s, r);
+        read_back(s, done);
         s->budget_blocked = false;                          // after the simulations went back to the pool
         std::lock_guard<std::mutex> lk(s->mu);
         s->t_readback += now_s() - t0;
@@ -457,26 +560,39 @@ pmk::FieldDesc field_desc(const pm_sim* h, int rho_index) {
     return d;
 }
 
+// Device descriptor area, and two pinned host staging buffers for it used in
+// turn: a plan build reuses a staging buffer only after the copy made from
+// it two builds earlier has completed (its event).
 bool ensure_scratch(pm_server* s, size_t bytes) {
     if (bytes <= s->dscratch_bytes) return true;
-    if (s->dscratch) (void)hipFree(s->dscratch);
-    if (s->hscratch) (void)hipHostFree(s->hscratch);
+    for (int i = 0; i < 2; ++i)
+        if (s->hscratch_done[i]) (void)hipEventSynchronize(s->hscratch_done[i]);
+    if (s->dscratch) (void)hipFree(s->dscratch);       // device-wide synchronisation: no round is in flight after
+    for (void*& hb : s->hscratch)
+        if (hb) { (void)hipHostFree(hb); hb = nullptr; }
     s->dscratch = nullptr;
-    s->hscratch = nullptr;
     s->dscratch_bytes = 0;
     size_t b = 1 << 16;
     while (b < bytes) b <<= 1;
     if (hipMalloc(&s->dscratch, b) != hipSuccess) return false;
-    if (hipHostMalloc(&s->hscratch, b, hipHostMallocDefault) != hipSuccess) return false;
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc(&s->hscratch[i], b, hipHostMallocDefault) != hipSuccess) return false;
+        if (!s->hscratch_done[i] && hipEventCreateWithFlags(&s->hscratch_done[i], hipEventDisableTiming) != hipSuccess)
+            return false;
+    }
     s->dscratch_bytes = b;
     return true;
 }
 
 // Builds and uploads the descriptor tables for the batched members; tables
 // for step parity p address rho_buf[rho_cur ^ p] as the current lattice.
-pm_status build_plan(pm_server* s, const std::vector<Request*>& members) {
+// The replaced round graph may still run (the previous round): it is handed
+// to `retired` and destroyed when the round being launched has completed.
+// The descriptor copy is ordered after the batch stream's earlier rounds, so
+// rewriting the device area does not disturb them.
+pm_status build_plan(pm_server* s, const std::vector<Request*>& members, std::vector<hipGraphExec_t>* retired) {
     BatchPlan& P = s->plan;
-    if (P.graph) { (void)hipGraphExecDestroy(P.graph); P.graph = nullptr; }
+    if (P.graph) { retired->push_back(P.graph); P.graph = nullptr; }
     P.members = members;
     P.signature.clear();
     for (Request* r : members) P.signature.push_back(r->id);
@@ -509,13 +625,21 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members) {
         s->batch_err = "scratch allocation failed";
         return PM_ENOMEM;
     }
-    char* hb = static_cast<char*>(s->hscratch);
+    const int hi = s->hscratch_next;
+    s->hscratch_next ^= 1;
+    if (hipEventSynchronize(s->hscratch_done[hi]) != hipSuccess) {
+        s->batch_err = "descriptor staging wait failed";
+        return PM_EDEVICE;
+    }
+    char* hb = static_cast<char*>(s->hscratch[hi]);
     for (auto& g : P.ngroups)
         for (int p = 0; p < 2; ++p) {
             auto* d = reinterpret_cast<pmk::FieldDesc*>(hb + g.fdesc_off[p]);
             for (size_t k = 0; k < g.idx.size(); ++k) {
                 const pm_sim* h = members[g.idx[k]]->sim;
                 d[k] = field_desc(h, h->rho_cur ^ p);
+                // the row R2C pass zeroes the next deposit target as it reads the current lattice
+                d[k].rho_zero = h->rho_buf[h->rho_cur ^ p ^ 1];
             }
         }
     for (auto& g : P.mgroups) {
@@ -557,8 +681,8 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members) {
             P.max_fill = std::max(P.max_fill, b);
         }
     }
-    if (hipMemcpyAsync(s->dscratch, s->hscratch, off, hipMemcpyHostToDevice, s->batch_stream) != hipSuccess ||
-        hipStreamSynchronize(s->batch_stream) != hipSuccess) {
+    if (hipMemcpyAsync(s->dscratch, hb, off, hipMemcpyHostToDevice, s->batch_stream) != hipSuccess ||
+        hipEventRecord(s->hscratch_done[hi], s->batch_stream) != hipSuccess) {
         s->batch_err = "descriptor upload failed";
         return PM_EDEVICE;
     }
@@ -570,9 +694,6 @@ pm_status launch_batched_steps(pm_server* s, int steps) {
     char* db = static_cast<char*>(s->dscratch);
     for (int st = 0; st < steps; ++st) {
         const int p = st & 1;
-        if (pmk::fill_zero_batched(reinterpret_cast<pmk::FillDesc*>(db + P.fill_off[p]), (int)P.members.size(),
-                                   P.max_fill, s->batch_stream) != hipSuccess)
-            return PM_EDEVICE;
         for (auto& g : P.ngroups) {
             const pm_sim* h0 = P.members[g.idx[0]]->sim;
             const auto* d = reinterpret_cast<const pmk::FieldDesc*>(db + g.fdesc_off[p]);
@@ -605,12 +726,13 @@ void advance_batched_state(pm_sim* h, int steps) {
     h->steps_since_sort = std::max(0, h->resort_interval - 1);
 }
 
-pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, int steps) {
+pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, int steps,
+                            std::vector<hipGraphExec_t>* retired) {
     std::vector<int64_t> sig;
     for (Request* r : members) sig.push_back(r->id);
     pm_status st = PM_OK;
     if (sig != s->plan.signature || s->plan.members.empty()) {
-        if ((st = build_plan(s, members)) != PM_OK) return st;
+        if ((st = build_plan(s, members, retired)) != PM_OK) return st;
     }
     if (s->cfg.use_graphs) {
         if (!s->plan.graph) {
@@ -695,82 +817,142 @@ double round_cost(const Request* r, int k) {
     return (double)k * (36.0 * (double)r->n + 100.0 * n * n + 2e6);
 }
 
-// One round: every admitted request advances min(remaining, round_steps)
-// steps (all remaining steps in PLAIN mode); small ones in batched launches,
-// the others over the lanes, longest first onto the least loaded lane.
-void run_round(pm_server* s) {
+hipEvent_t take_event(pm_server* s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess ? e : nullptr;
+}
+
+// Enqueues one round: every active request advances min(remaining,
+// round_steps) steps (all remaining steps in PLAIN mode); small ones in the
+// batched launches, the others on their lane streams.  Nothing waits here:
+// the round's end is an event per stream, completed by complete_round once
+// the next round has been enqueued behind it, so the device never drains
+// between rounds and a lane that finishes early starts its next round at
+// once.  A request stays on the lane it was first given (least loaded by
+// simulated bytes), so its rounds are ordered by that stream; after a batched
+// round it waits for the batch stream.
+void launch_round(pm_server* s) {
     const int R = s->cfg.mode == PM_SERVE_PLAIN ? 0 : std::max(2, s->cfg.round_steps & ~1);
-    std::vector<Request*> batched, others;
-    std::map<Request*, int> advance;
+    InFlight f;
+    f.t_launch = now_s();
+    std::vector<Request*> batched, others, keep;
+    std::map<Request*, int> adv;
     for (Request* r : s->active) {
-        const int rem = r->steps - r->done_steps;
+        const int rem = r->steps - r->launched_steps;
         const int k = R == 0 ? rem : std::min(rem, R);
-        advance[r] = k;
+        adv[r] = k;
         if (R > 0 && rem >= R && r->batched) batched.push_back(r);
         else others.push_back(r);
     }
-    std::sort(others.begin(), others.end(),
-              [&](Request* a, Request* b) { return round_cost(a, advance[a]) > round_cost(b, advance[b]); });
-    double load[kLanes] = {};
-    const double t0 = now_s();
     std::vector<Request*> bad;
+    bool batch_used = false, lane_used[kLanes] = {};
     if (!batched.empty()) {
         s->batch_err = "batched launch failed";
-        if (run_batched_round(s, batched, R) != PM_OK) {
+        if (run_batched_round(s, batched, R, &f.retired) != PM_OK) {
             for (Request* r : batched) { r->msg = s->batch_err; bad.push_back(r); }
             s->plan.signature.clear();
         }
+        batch_used = true;
+        (void)hipEventRecord(s->batch_done, s->batch_stream);
     }
     for (Request* r : others) {
-        const int l = (int)(std::min_element(load, load + kLanes) - load);
-        load[l] += round_cost(r, advance[r]);
-        r->sim->stream = s->lanes[l];
-        const int rem = r->steps - r->done_steps;
-        const pm_status st = (R > 0 && rem >= R) ? run_large_round(s, r, R) : pm_step(r->sim, advance[r]);
+        if (r->lane < 0) {
+            r->lane = (int)(std::min_element(s->lane_load, s->lane_load + kLanes) - s->lane_load);
+            r->lane_cost = round_cost(r, std::max(1, R)) * (double)(r->steps - r->launched_steps);
+            s->lane_load[r->lane] += r->lane_cost;
+        }
+        hipStream_t q = s->lanes[r->lane];
+        if (r->last_batched) (void)hipStreamWaitEvent(q, s->batch_done, 0);
+        r->sim->stream = q;
+        lane_used[r->lane] = true;
+        const int rem = r->steps - r->launched_steps;
+        const pm_status st = (R > 0 && rem >= R) ? run_large_round(s, r, R) : pm_step(r->sim, adv[r]);
         if (st != PM_OK) { r->msg = r->sim->err; bad.push_back(r); }
     }
-    const double t_l = now_s();
-    if (!batched.empty() && hipStreamSynchronize(s->batch_stream) != hipSuccess) {
-        for (Request* r : batched) { r->msg = "device error in batched round"; bad.push_back(r); }
-        s->plan.signature.clear();
-    }
-    for (hipStream_t q : s->lanes)
-        if (hipStreamSynchronize(q) != hipSuccess)
-            for (Request* r : others) { r->msg = "device error"; bad.push_back(r); }
-    const double t1 = now_s();
-    std::vector<Request*> keep, out;
+    for (Request* r : batched) r->last_batched = true;
+    for (Request* r : others) r->last_batched = false;
+    if (batch_used && (f.ev[0] = take_event(s))) (void)hipEventRecord(f.ev[0], s->batch_stream);
+    for (int l = 0; l < kLanes; ++l)
+        if (lane_used[l] && (f.ev[1 + l] = take_event(s))) (void)hipEventRecord(f.ev[1 + l], s->lanes[l]);
     for (Request* r : s->active) {
-        if (std::find(bad.begin(), bad.end(), r) != bad.end()) {
+        const bool failed = std::find(bad.begin(), bad.end(), r) != bad.end();
+        if (failed) {
             r->failed = true;
             if (r->batched) s->plan.signature.clear();
-            out.push_back(r);
+            f.completing.push_back(r);
             continue;
         }
-        const int k = advance[r];
-        r->done_steps += k;
-        if (r->done_steps >= r->steps) {
+        const int k = adv[r];
+        r->launched_steps += k;
+        f.advance.push_back({r, k, r->last_batched ? 0 : 1 + r->lane});
+        if (r->launched_steps >= r->steps) {
             if (r->batched) s->plan.signature.clear();
-            out.push_back(r);
+            f.completing.push_back(r);
         } else {
             keep.push_back(r);
         }
     }
     s->active.swap(keep);
+    const double t_l = now_s();
     std::lock_guard<std::mutex> lk(s->mu);
-    const float lat_round = (float)((t1 - t0) * 1e3);
-    for (const auto& [r, k] : advance) {
+    s->rounds += 1;
+    s->t_launch += t_l - f.t_launch;
+    s->n_active = (int64_t)s->active.size();
+    s->inflight.push_back(std::move(f));
+}
+
+// Waits for the oldest launched round, accounts its steps and hands the
+// requests that reached their step count (or failed) to the read-back thread.
+void complete_round(pm_server* s) {
+    InFlight f = std::move(s->inflight.front());
+    s->inflight.pop_front();
+    const double t0 = now_s();
+    bool err[1 + kLanes] = {};
+    for (int i = 0; i < 1 + kLanes; ++i)
+        if (f.ev[i]) {
+            err[i] = hipEventSynchronize(f.ev[i]) != hipSuccess;
+            s->event_pool.push_back(f.ev[i]);
+        }
+    const double t1 = now_s();
+    for (hipGraphExec_t g : f.retired) (void)hipGraphExecDestroy(g);
+    for (const InFlight::Advance& a : f.advance) {
+        Request* r = a.r;
+        if (err[a.stream] && !r->failed) {
+            r->failed = true;
+            r->msg = "device error";
+            if (std::find(f.completing.begin(), f.completing.end(), r) == f.completing.end()) {
+                // still in the active list: drop it there, finish it with this round
+                s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
+                f.completing.push_back(r);
+            }
+        }
+    }
+    // per-step latency: this round's share of the time since the previous
+    // round completed (or since its launch, if later)
+    const double dt = t1 - std::max(f.t_launch, s->t_last_complete);
+    s->t_last_complete = t1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const InFlight::Advance& a : f.advance) {
+        Request* r = a.r;
+        const int k = a.steps;
         if (r->failed || k <= 0) continue;
-        const float lat = lat_round / (float)k;
+        r->done_steps += k;
+        const float lat = (float)(dt * 1e3 / (double)k);
         for (int q = 0; q < k; ++q) s->step_lat_ms.push_back(lat);
         s->particle_steps += (double)r->n * k;
         s->steps_done += k;
     }
-    s->rounds += 1;
-    s->t_launch += t_l - t0;
-    s->t_wait += t1 - t_l;
+    for (Request* r : f.completing)
+        if (r->lane >= 0) s->lane_load[r->lane] -= r->lane_cost;
+    s->t_wait += t1 - t0;
     s->n_active = (int64_t)s->active.size();
-    if (!out.empty()) {
-        s->finished.insert(s->finished.end(), out.begin(), out.end());
+    if (!f.completing.empty()) {
+        s->finished.insert(s->finished.end(), f.completing.begin(), f.completing.end());
         s->cv_io.notify_all();
     }
 }
@@ -782,10 +964,10 @@ void worker_main(pm_server* s) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
-                return !s->ready.empty() || !s->active.empty() ||
+                return !s->ready.empty() || !s->active.empty() || !s->inflight.empty() ||
                        (s->stop && s->queue.empty() && s->slots_used == 0);
             });
-            if (s->ready.empty() && s->active.empty()) break;   // stopping, nothing left
+            if (s->ready.empty() && s->active.empty() && s->inflight.empty()) break;   // stopping, nothing left
             fresh.swap(s->ready);
             s->active.insert(s->active.end(), fresh.begin(), fresh.end());
             s->n_active = (int64_t)s->active.size();
@@ -797,7 +979,9 @@ void worker_main(pm_server* s) {
             for (hipStream_t q : s->lanes) (void)hipStreamWaitEvent(q, r->admitted, 0);
         }
         fresh.clear();
-        run_round(s);
+        if (!s->active.empty()) launch_round(s);
+        // keep one round queued behind the one running
+        while (s->inflight.size() > 1 || (s->active.empty() && !s->inflight.empty())) complete_round(s);
     }
 }
 
@@ -818,6 +1002,7 @@ void pm_server_config_default(pm_server_config* c) {
     c->batch_max_lattice = 2048;
     c->batch_max_particles = 1 << 17;
     c->device = 0;
+    c->pinned_staging_mib = 1024;
 }
 
 int32_t pm_plan_admission(const int64_t* need_bytes, int32_t n_queued, int64_t free_bytes, int32_t max_new) {
@@ -838,8 +1023,18 @@ static void free_server(pm_server* s) {
     for (pm_sim* h : s->pool) pm_destroy(h);
     for (auto& kv : s->spectra) (void)hipFree(kv.second);
     if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
+    for (auto& f : s->inflight) {
+        for (hipGraphExec_t g : f.retired) (void)hipGraphExecDestroy(g);
+        for (hipEvent_t e : f.ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->batch_done) (void)hipEventDestroy(s->batch_done);
     if (s->dscratch) (void)hipFree(s->dscratch);
-    if (s->hscratch) (void)hipHostFree(s->hscratch);
+    for (int i = 0; i < 2; ++i) {
+        if (s->hscratch[i]) (void)hipHostFree(s->hscratch[i]);
+        if (s->hscratch_done[i]) (void)hipEventDestroy(s->hscratch_done[i]);
+    }
     for (hipStream_t* q : {&s->batch_stream, &s->io_stream, &s->rb_stream})
         if (*q) (void)hipStreamDestroy(*q);
     for (hipStream_t& q : s->lanes)
@@ -850,7 +1045,7 @@ static void free_server(pm_server* s) {
 pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     if (!cfg || !out) return PM_EINVAL;
     *out = nullptr;
-    if (cfg->hbm_budget_bytes <= 0 || cfg->max_active <= 0 || cfg->round_steps <= 0 ||
+    if (cfg->hbm_budget_bytes <= 0 || cfg->max_active <= 0 || cfg->round_steps <= 0 || cfg->pinned_staging_mib < 0 ||
         (cfg->mode != PM_SERVE_PLAIN && cfg->mode != PM_SERVE_BATCHED))
         return PM_EINVAL;
     if (pm_device_count() <= cfg->device || cfg->device < 0) return PM_ENODEV;
@@ -858,9 +1053,14 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     if (!s) return PM_ENOMEM;
     s->cfg = *cfg;
     bool ok = hipSetDevice(cfg->device) == hipSuccess;
+    if (ok && !s->arena.init((size_t)std::max(0, cfg->pinned_staging_mib) << 20)) {
+        free_server(s);
+        return PM_ENOMEM;
+    }
     for (hipStream_t* q : {&s->batch_stream, &s->io_stream, &s->rb_stream})
         ok = ok && hipStreamCreateWithFlags(q, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& q : s->lanes) ok = ok && hipStreamCreateWithFlags(&q, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&s->batch_done, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         free_server(s);
         return PM_EDEVICE;
@@ -894,11 +1094,20 @@ pm_status pm_server_submit(pm_server_handle s, const pm_config* cfg, const float
     r->cfg = c;
     r->n = n;
     r->steps = steps;
-    r->x.assign(x, x + n);
-    r->y.assign(y, y + n);
-    r->vx.assign(vx, vx + n);
-    r->vy.assign(vy, vy + n);
-    r->m.assign(m, m + n);
+    const size_t nb = (size_t)n * sizeof(float);
+    float* blk = n > 0 ? reinterpret_cast<float*>(s->arena.take(5 * nb, &r->pinned_cls)) : nullptr;
+    if (blk) {
+        r->arena = &s->arena;
+        r->pinned = reinterpret_cast<char*>(blk);
+    } else {
+        r->pageable.resize((size_t)5 * n);
+        blk = r->pageable.data();
+    }
+    const float* src[5] = {x, y, vx, vy, m};
+    for (int k = 0; k < 5; ++k) {
+        r->arr[k] = blk + (size_t)k * n;
+        if (n > 0) std::memcpy(r->arr[k], src[k], nb);
+    }
     // the shift pm_load would pick, computed here on the caller's thread so
     // that admission does not make a pass over the masses
     if (r->cfg.mass_shift <= 0) r->cfg.mass_shift = pmi::auto_shift(m, n);
@@ -950,12 +1159,15 @@ pm_status pm_server_wait(pm_server_handle s, int64_t id, float* x, float* y, flo
         s->reqs.erase(it);
         return PM_EDEVICE;
     }
-    const size_t b = (size_t)r->n * sizeof(float);
-    if (x) std::memcpy(x, r->x.data(), b);
-    if (y) std::memcpy(y, r->y.data(), b);
-    if (vx) std::memcpy(vx, r->vx.data(), b);
-    if (vy) std::memcpy(vy, r->vy.data(), b);
+    // copy the result out without holding the server lock (the worker and
+    // admission threads take it every round)
+    std::unique_ptr<Request> own = std::move(it->second);
     s->reqs.erase(it);
+    lk.unlock();
+    const size_t b = (size_t)own->n * sizeof(float);
+    float* dst[4] = {x, y, vx, vy};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k] && b) std::memcpy(dst[k], own->arr[k], b);
     return PM_OK;
 }
 

@@ -98,6 +98,9 @@ pm_status hip_fail(pm_sim* h, hipError_t e, const char* what);
 pm_status validate(const pm_config* c, std::string* why);
 int64_t align256(int64_t b);
 pm_status upload_field_desc(pm_sim* h);
+// pm_read without the final wait: the copies into the caller's buffers are
+// enqueued on h->stream (complete once the stream is synchronised)
+pm_status read_async(pm_sim* h, float* x, float* y, float* vx, float* vy);
 pm_status sort_now(pm_sim* h, bool counted);
 // Fixed-point shift pm_load picks for these masses (cfg.mass_shift 0).
 int auto_shift(const float* m, int64_t n);

@@ -330,7 +330,8 @@ class ServerConfig(ctypes.Structure):
         ("batch_max_lattice", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("batch_max_particles", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("pinned_staging_mib", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

@@ -47,7 +47,11 @@ typedef struct pm_server_config {
     int32_t device;
     int32_t batch_max_particles; /* ... with at most this many particles (default 131072) run in
                                   * batched launches (BATCHED) */
-    int32_t reserved[7];
+    int32_t pinned_staging_mib;  /* pinned host memory (MiB, taken at creation; default 1024) that
+                                  * holds submitted arrays until their results are collected:
+                                  * uploads and read-backs from it are asynchronous DMA.  A request
+                                  * that finds no room uses pageable memory.  0 = none. */
+    int32_t reserved[6];
 } pm_server_config;
 
 typedef struct pm_server_stats {

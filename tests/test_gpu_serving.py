@@ -45,16 +45,18 @@ def mix():
     return reqs, want
 
 
-@pytest.mark.parametrize("mode,graphs,batch_max", [
-    (pm.SERVE_BATCHED, 1, 512),
-    (pm.SERVE_BATCHED, 0, 512),
-    (pm.SERVE_BATCHED, 1, 0),       # no batched launches: every request on its own stream
-    (pm.SERVE_PLAIN, 0, 512),
+@pytest.mark.parametrize("mode,graphs,batch_max,pinned_mib", [
+    (pm.SERVE_BATCHED, 1, 512, 1024),
+    (pm.SERVE_BATCHED, 0, 512, 1024),
+    (pm.SERVE_BATCHED, 1, 0, 1024),       # no batched launches: every request on its own stream
+    (pm.SERVE_PLAIN, 0, 512, 1024),
+    (pm.SERVE_BATCHED, 1, 512, 0),        # request arrays in pageable memory
+    (pm.SERVE_BATCHED, 1, 512, 2),        # 2 MiB of pinned staging: some requests pinned, the rest pageable
 ])
-def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max):
+def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max, pinned_mib):
     reqs, want = mix
-    print(f"serving mode={mode} graphs={graphs} batch_max_lattice={batch_max}", flush=True)
-    with pm.Server(mode=mode, use_graphs=graphs, batch_max_lattice=batch_max,
+    print(f"serving mode={mode} graphs={graphs} batch_max_lattice={batch_max} pinned={pinned_mib} MiB", flush=True)
+    with pm.Server(mode=mode, use_graphs=graphs, batch_max_lattice=batch_max, pinned_staging_mib=pinned_mib,
                    hbm_budget_bytes=8 << 30) as srv:
         ids = [srv.submit(c, *a, steps=st) for (c, a), (_, _, _, st, _) in zip(reqs, MIX)]
         got = [srv.wait(i, timeout_s=120) for i in ids]

@@ -80,6 +80,7 @@ def test_server_config_defaults():
     c = pm.default_server_config()
     assert c.mode == pm.SERVE_BATCHED and c.round_steps == 8 and c.use_graphs == 1
     assert c.hbm_budget_bytes == 200 << 30 and c.max_active > 1
+    assert c.pinned_staging_mib == 1024
 
 
 def test_plan_admission_is_strict_fifo():

@@ -85,13 +85,14 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(n, mode, dt, arrs, steps, threads):
-    """The CPU oracle step (int64 deposit, FP64 FFT field, FP32 push) on the
-    same particles, timed on this host's cores."""
+def cpu_baseline(n, mode, dt, arrs, steps, threads, shift):
+    """The FP32 CPU step (oracle/pm_cpu32.c: int64 deposit into per-thread
+    lattices allocated once, FP32 FFT convolution, the reference's FP32
+    stencil and push) on the same particles, timed on this host's cores."""
     from oracle import oracle as orc
     orc.set_threads(threads)
     x, y, vx, vy, m = arrs
-    s = orc.MixedStepper(n, mode, np.float32(dt), 1.0, 1.0, 1e-3, 0, 30, x, y, vx, vy, m)
+    s = orc.Cpu32Stepper(n, mode, np.float32(dt), 1.0, 1.0, 1e-3, 0, shift, x, y, vx, vy, m)
     s.step(1)                         # warm-up (page-in, thread start)
     t0 = time.perf_counter()
     s.step(steps)
@@ -133,7 +134,7 @@ def precision_report(seed):
     print(f"# precision: ics {npart} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
     T = orc.transfer(n, pm.CIC, 1.0, 1e-3)
     out = {"workload": f"plummer_binary_{n}_{npart // 1_000_000}M_cic", "lattice": n, "particles": npart, "dt": dt}
-    for name, prec in (("fp32", pm.FFT_FP32), ("bf16x3", pm.FFT_BF16X3)):
+    for name, prec in (("fp32", pm.FFT_FP32), ("bf16x3", pm.FFT_BF16X3), ("bf16x6", pm.FFT_BF16X6)):
         cfg = pm.default_config(n_lattice=n, n_particles=npart, mode=pm.CIC, dt=dt, gravity=1.0, green_origin=1.0,
                                 deconv_eps=1e-3, precision=prec)
         rows = []
@@ -283,10 +284,17 @@ def serving_naive_loop(pm, reqs, arrays):
                     "before timing, as for the server modes"}
 
 
+def shift_of(m):
+    """The fixed-point shift the engine picks for these masses: floor(61 -
+    log2(sum |m|)), clamped to [0, 62]."""
+    s = float(np.sum(np.abs(m.astype(np.float64))))
+    return 40 if not s > 0 else max(0, min(62, int(np.floor(61.0 - np.log2(s)))))
+
+
 def serving_cpu_baseline(ics, reqs, templates, steps=5):
-    """The one-at-a-time loop on the host cores with the oracle's step (int64
-    deposit, FP64 FFT field, FP32 push) over a subset of the requests, a few
-    steps each (a bounded sample)."""
+    """The one-at-a-time loop on the host cores with the FP32 CPU step
+    (oracle/pm_cpu32.c) over a subset of the requests, a few steps each (a
+    bounded sample, set-up included)."""
     from oracle import oracle as orc
     threads = os.cpu_count()
     orc.set_threads(threads)
@@ -295,14 +303,14 @@ def serving_cpu_baseline(ics, reqs, templates, steps=5):
     for r in reqs:
         x, y, vx, vy, m, g = ics.ensemble_request(templates[r["kind"]], r["n_lattice"], r["n"], r["seed"])
         t0 = time.perf_counter()
-        st = orc.MixedStepper(r["n_lattice"], 1, np.float32(SERVING["dt"]), np.float32(g), 1.0, 1e-3, 0, 30,
-                              x, y, vx, vy, m)
+        st = orc.Cpu32Stepper(r["n_lattice"], 1, np.float32(SERVING["dt"]), np.float32(g), 1.0, 1e-3, 0,
+                              shift_of(m), x, y, vx, vy, m)
         st.step(steps)
         el += time.perf_counter() - t0
         psteps += steps * r["n"]
     return {"value": psteps / el, "unit": "particle-steps/s", "cores": orc.num_threads(), "kind": "port",
             "sample": f"first {len(reqs)} requests of the ensemble, {steps} steps each (set-up included), "
-                      f"{el:.1f}s; oracle/pm_oracle.c, host {cpu_model()}"}
+                      f"{el:.1f}s; oracle/pm_cpu32.c (FP32 CPU step), host {cpu_model()}"}
 
 
 def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated", "poisson"), use_graphs=1):
@@ -356,7 +364,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="colliding_disks_2048_100M_cic", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=30, help="timed oracle steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=60,
+                    help="timed FP32 CPU steps for cpu_baseline at 1e8 particles, scaled by the count (0 = skip)")
     ap.add_argument("--breakdown-steps", type=int, default=16,
                     help="steps of the profiled pass after the timed region that times the non-step kernels")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
@@ -369,7 +378,8 @@ def main():
     ap.add_argument("--serving", choices=("on", "off", "only"), default="on",
                     help="serving ensemble leg (plain vs batched server, saturated and Poisson arrivals)")
     ap.add_argument("--precision-report", action="store_true",
-                    help="only the 4096^2 Plummer-binary FP32 vs split-BF16 FFT comparison (error vs FP64 per step)")
+                    help="only the 4096^2 Plummer-binary FP32 vs split-BF16 (bf16x3, bf16x6) FFT comparison "
+                         "(error vs FP64 per step)")
     ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
     ap.add_argument("--serving-modes", default="plain,batched,naive,cpu")
     ap.add_argument("--serving-arrivals", default="saturated,poisson")
@@ -427,6 +437,7 @@ def main():
     sim.profile(False)
     prof = {k: (prof[k] if k in step_kernels else prof_rest[k]) for k in prof}
     diag = sim.diagnostics()
+    shift = sim.info().mass_shift
 
     kernels = {}
     for name, (ms, cnt) in prof.items():
@@ -505,12 +516,14 @@ def main():
     if args.cpu_steps > 0:
         threads = args.cpu_threads or os.cpu_count()
         cpu_steps = max(1, int(args.cpu_steps * 100_000_000 // np_))     # bounded sample: ~the same CPU time
-        print(f"# cpu baseline: {cpu_steps} oracle steps on {threads} threads", file=sys.stderr, flush=True)
-        v, el_cpu, used = cpu_baseline(n, mode, dt, arrs, cpu_steps, threads)
+        print(f"# cpu baseline: {cpu_steps} FP32 CPU steps on {threads} threads", file=sys.stderr, flush=True)
+        v, el_cpu, used = cpu_baseline(n, mode, dt, arrs, cpu_steps, threads, shift)
         out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": used, "kind": "port",
                                "sample": f"{cpu_steps} full steps of the same workload ({np_} particles, "
-                                         f"{n}^2) after 1 warm-up step, {el_cpu:.1f}s; oracle/pm_oracle.c "
-                                         f"(int64 deposit, FP64 FFT field, FP32 push), host {cpu_model()}"}
+                                         f"{n}^2) after 1 warm-up step, {el_cpu:.1f}s; oracle/pm_cpu32.c "
+                                         f"(int64 deposit into per-thread lattices allocated once, FP32 FFT "
+                                         f"convolution, FP32 reference stencil and push; OpenMP), "
+                                         f"host {cpu_model()}"}
     else:
         out["cpu_baseline"] = None
     if args.serving == "on":

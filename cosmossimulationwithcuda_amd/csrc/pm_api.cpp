@@ -49,9 +49,12 @@ pm_status validate(const pm_config* c, std::string* why) {
     if (!(c->deconv_eps >= 0.0) || !std::isfinite(c->green_origin)) { *why = "bad deconv_eps or green_origin"; return PM_EINVAL; }
     if (c->wrap_mode != PM_WRAP_KEEP_VELOCITY && c->wrap_mode != PM_WRAP_REFERENCE) { *why = "bad wrap_mode"; return PM_EINVAL; }
     if (c->force_mode != PM_FORCE_SPECTRAL && c->force_mode != PM_FORCE_STENCIL) { *why = "bad force_mode"; return PM_EINVAL; }
-    if (c->precision != PM_FFT_FP32 && c->precision != PM_FFT_BF16X3) { *why = "precision: unknown"; return PM_EINVAL; }
-    if (c->precision == PM_FFT_BF16X3 && (c->n_lattice < 256 || c->n_lattice > 8192)) {
-        *why = "precision PM_FFT_BF16X3 needs 256 <= n_lattice <= 8192";
+    if (c->precision != PM_FFT_FP32 && c->precision != PM_FFT_BF16X3 && c->precision != PM_FFT_BF16X6) {
+        *why = "precision: unknown";
+        return PM_EINVAL;
+    }
+    if (c->precision != PM_FFT_FP32 && (c->n_lattice < 256 || c->n_lattice > 8192)) {
+        *why = "the split-BF16 precisions need 256 <= n_lattice <= 8192";
         return PM_EINVAL;
     }
     if (c->mass_shift < 0 || c->mass_shift > 62) { *why = "mass_shift must be in [0, 62]"; return PM_EINVAL; }
@@ -81,9 +84,10 @@ int auto_resort_interval(int mode, int64_t np, int n) {
 }
 
 pmk::FftPlan plan_for(const pm_config& c) {
-    if (c.precision == PM_FFT_BF16X3) {
+    if (c.precision != PM_FFT_FP32) {
         pmk::FftPlan p;
-        p.rp = 1; p.cb = 1; p.radix = 16;
+        p.rp = 1; p.cb = 1; p.radix = c.precision == PM_FFT_BF16X6 ? 17 : 16;
+        p.it = pmk::pipe_plan_compiled(c.n_lattice, p.radix, 2) ? 2 : 1;
         return p;
     }
     return pmk::default_plan(c.n_lattice);
@@ -930,8 +934,9 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         // batch or radix has no prefetching variant
         if (k != "fft_it" && p.it != 1 && !pmk::plan_valid(h->cfg.n_lattice, p)) p.it = 1;
         if (!pmk::plan_valid(h->cfg.n_lattice, p)) return fail(h, PM_EINVAL, "FFT plan not available for this lattice");
-        if ((p.radix == 16) != (h->cfg.precision == PM_FFT_BF16X3))
-            return fail(h, PM_EINVAL, "fft_radix 16 is the PM_FFT_BF16X3 precision; 4, 8 or 64 are FP32");
+        if (p.radix != plan_for(h->cfg).radix && ((p.radix >= 16 && p.radix <= 17) || h->cfg.precision != PM_FFT_FP32))
+            return fail(h, PM_EINVAL,
+                        "fft_radix 16 is the PM_FFT_BF16X3 precision, 17 PM_FFT_BF16X6; 4, 8 or 64 are FP32");
         h->plan = p;
         return PM_OK;
     }

@@ -335,6 +335,7 @@ hipError_t r2c_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, hip
     if constexpr (rp_ok<N, RPV, RV>()) if (p.rp == RPV && p.radix == RV) return r2c_t<N, RPV, RV>(d, count, tw, s);
     PM_R2C(1, 4) PM_R2C(2, 4) PM_R2C(4, 4) PM_R2C(1, 8) PM_R2C(2, 8) PM_R2C(4, 8)
     if constexpr (N >= 256) if (p.rp == 1 && p.radix == 16) return r2c_t<N, 1, 16>(d, count, tw, s);
+    if constexpr (N >= 256) if (p.rp == 1 && p.radix == 17) return r2c_t<N, 1, 17>(d, count, tw, s);
 #undef PM_R2C
     return hipErrorInvalidValue;
 }
@@ -345,6 +346,7 @@ hipError_t col_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, boo
     if constexpr (cb_ok<N, CBV, RV>()) if (p.cb == CBV && p.radix == RV) return col_t<N, CBV, RV>(d, count, tw, force, s);
     PM_COL(1, 4) PM_COL(2, 4) PM_COL(4, 4) PM_COL(8, 4) PM_COL(1, 8) PM_COL(2, 8) PM_COL(4, 8) PM_COL(8, 8)
     if constexpr (N >= 256) if (p.cb == 1 && p.radix == 16) return col_t<N, 1, 16>(d, count, tw, force, s);
+    if constexpr (N >= 256) if (p.cb == 1 && p.radix == 17) return col_t<N, 1, 17>(d, count, tw, force, s);
 #undef PM_COL
     return hipErrorInvalidValue;
 }
@@ -355,6 +357,7 @@ hipError_t c2r_n(const FieldDesc* d, int count, const float2* tw, FftPlan p, boo
     if constexpr (rp_ok<N, RPV, RV>()) if (p.rp == RPV && p.radix == RV) return c2r_t<N, RPV, RV>(d, count, tw, force, s);
     PM_C2R(1, 4) PM_C2R(2, 4) PM_C2R(4, 4) PM_C2R(1, 8) PM_C2R(2, 8) PM_C2R(4, 8)
     if constexpr (N >= 256) if (p.rp == 1 && p.radix == 16) return c2r_t<N, 1, 16>(d, count, tw, force, s);
+    if constexpr (N >= 256) if (p.rp == 1 && p.radix == 17) return c2r_t<N, 1, 17>(d, count, tw, force, s);
 #undef PM_C2R
     return hipErrorInvalidValue;
 }
@@ -390,7 +393,7 @@ hipError_t stencil_n(const FieldDesc* d, int count, hipStream_t s) {
 bool plan_valid(int n, FftPlan p) {
     if (p.radix == kWaveRadix) return p.rp == 1 && p.cb == 1 && p.it == 1 && wave_plan_compiled(n);
     if (p.it != 1) return p.rp == 1 && p.cb == 1 && pipe_plan_compiled(n, p.radix, p.it);
-    if (p.radix == 16) return p.rp == 1 && p.cb == 1 && n >= 256 && n <= 8192;
+    if (p.radix == 16 || p.radix == 17) return p.rp == 1 && p.cb == 1 && n >= 256 && n <= 8192;
     if (p.radix != 4 && p.radix != 8) return false;
     if (p.rp != 1 && p.rp != 2 && p.rp != 4) return false;
     if (p.cb != 1 && p.cb != 2 && p.cb != 4 && p.cb != 8) return false;
@@ -431,7 +434,7 @@ hipError_t col_conv(int n, const FieldDesc* d, int count, const float2* tw, FftP
 hipError_t row_c2r(int n, const FieldDesc* d, int count, const float2* tw, FftPlan p, bool with_force,
                    hipStream_t s) {
     if (p.radix == kWaveRadix) return wave_pass(2, n, d, count, tw, with_force, s);
-    if (p.it != 1) return row_c2r_pipe(n, d, count, tw, p, with_force, s);
+    if (p.it != 1 && p.radix < 16) return row_c2r_pipe(n, d, count, tw, p, with_force, s);
     PM_DISPATCH_N(n, (c2r_n<NN>(d, count, tw, p, with_force, s)));
 }
 

@@ -63,11 +63,11 @@ constexpr int first_radix_ns(int n, int r) {
     return ilog2c(n) % ilog2c(r) ? 1 << (ilog2c(n) % ilog2c(r)) : 1;
 }
 constexpr bool stage_table(int n, int r, int ns) {
-    return n >= 1024 && r != 16 && ns > 1 && n / (r * ns) > 1 && (r - 1) * ns <= 1024;
+    return n >= 1024 && r < 16 && ns > 1 && n / (r * ns) > 1 && (r - 1) * ns <= 1024;
 }
 constexpr int stage_table_off(int n, int r, int ns) {
     int off = n / 4;
-    if (r == 16) return off;
+    if (r >= 16) return off;
     for (int s = first_radix_ns(n, r); s < ns; s *= r)
         if (stage_table(n, r, s)) off += (r - 1) * s;
     return off;
@@ -183,7 +183,7 @@ __device__ __forceinline__ int row_group() {
 // solve 0.36-0.41 -> 0.34 ms), else 256 (512 is slower at 2048 with two row
 // pairs per block; the matrix-core radix-16 stages are written for 256).
 template <int N, int RADIX>
-constexpr int fft_threads() { return N >= 4096 && RADIX != 16 ? 512 : kBlock; }
+constexpr int fft_threads() { return N >= 4096 && RADIX < 16 ? 512 : kBlock; }
 
 template <int N, int R, int NS, int BT>
 __device__ __forceinline__ void load_stage_tables(float2* q, const float2* __restrict__ tw) {
@@ -204,7 +204,7 @@ __device__ __forceinline__ void load_stage_tables(float2* q, const float2* __res
 template <int N, int R, int BT = kBlock>
 __device__ __forceinline__ void load_twiddles(float2* q, const float2* __restrict__ tw) {
     for (int e = threadIdx.x; e < N / 4; e += BT) q[e] = tw[e];
-    if constexpr (R != 16) load_stage_tables<N, R, first_radix_ns(N, R), BT>(q, tw);
+    if constexpr (R < 16) load_stage_tables<N, R, first_radix_ns(N, R), BT>(q, tw);
 }
 
 // One Stockham stage of radix R with sub-transform length NS over NFFT
@@ -280,15 +280,19 @@ __device__ __forceinline__ void stages_from(float2* buf, const float2* tw) {
 // NFFT in-place transforms of length N at buf + f*N (padded), radix RADIX
 // (4 or 8) after a leading stage for the leftover factor; tw is the LDS quarter
 // table.  Caller synchronises before; ends with a barrier.
-// ---- split-BF16 matrix-core radix-16 stages (precision variant) --------------
+// ---- split-BF16 matrix-core radix-16 stages (precision variants) -------------
 // A radix-16 stage over a group of 16 butterflies is one complex 16x16 matrix
 // product X = F * x (F[m][r] = exp(-+2 pi i m r / 16)), done as the real
 // product [Xr; Xi] = [[Fr, -Fi], [Fi, Fr]] [xr; xi] on the matrix cores with
-// v_mfma_f32_16x16x16_bf16: every FP32 operand is split into bf16 hi + lo
-// parts and each real product is formed as hi*hi + hi*lo + lo*hi with FP32
-// accumulation (the lo*lo term, ~2^-16 relative, is dropped).  Operand
-// layout (16x16x16): lane l holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16]
-// for j < 4, and receives D[4(l/16)+i][l%16] for i < 4.
+// v_mfma_f32_16x16x16_bf16, FP32 accumulation.  Every FP32 operand is split
+// into bf16 parts:
+//   S3 = false (bf16x3): hi + lo, products hi*hi + hi*lo + lo*hi; the lo*lo
+//     term (~2^-16 relative) is dropped and each operand keeps ~17 bits;
+//   S3 = true (bf16x6): hi + mid + lo (~25 bits), products hi*hi, hi*mid,
+//     mid*hi, hi*lo, lo*hi, mid*mid; the dropped terms are ~2^-24 relative.
+// Small products are accumulated first.  Operand layout (16x16x16): lane l
+// holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16] for j < 4, and receives
+// D[4(l/16)+i][l%16] for i < 4.
 typedef short v4bf __attribute__((ext_vector_type(4)));
 typedef float v4acc __attribute__((ext_vector_type(4)));
 
@@ -297,9 +301,16 @@ __device__ __forceinline__ short bf16_rne(float x) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (short)(u >> 16);
 }
+__device__ __forceinline__ float bf16_val(short b) { return __uint_as_float((unsigned)(unsigned short)b << 16); }
 __device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
     hi = bf16_rne(x);
-    lo = bf16_rne(x - __uint_as_float((unsigned)(unsigned short)hi << 16));
+    lo = bf16_rne(x - bf16_val(hi));
+}
+__device__ __forceinline__ void split3_bf16(float x, short& hi, short& mid, short& lo) {
+    hi = bf16_rne(x);
+    const float r1 = x - bf16_val(hi);
+    mid = bf16_rne(r1);
+    lo = bf16_rne(r1 - bf16_val(mid));
 }
 
 // cos(2 pi j / 16), j < 16
@@ -310,10 +321,10 @@ __constant__ float kCos16[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752
 
 // DFT-16 operands: [h][kh] = output half (re, im) x input half (re, im).
 struct Dft16Ops {
-    v4bf hi[2][2], lo[2][2];
+    v4bf hi[2][2], mid[2][2], lo[2][2];
 };
 
-template <bool INV>
+template <bool INV, bool S3>
 __device__ __forceinline__ void dft16_ops(Dft16Ops& A) {
     const int l = threadIdx.x & 63, m = l & 15, g = l >> 4;
 #pragma unroll
@@ -326,10 +337,16 @@ __device__ __forceinline__ void dft16_ops(Dft16Ops& A) {
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int kh = 0; kh < 2; ++kh) {
-                short a, b;
-                split_bf16(v[h][kh], a, b);
+                short a, b, c3 = 0;
+                if (S3) split3_bf16(v[h][kh], a, b, c3);
+                else split_bf16(v[h][kh], a, b);
                 A.hi[h][kh][j] = a;
-                A.lo[h][kh][j] = b;
+                if (S3) {
+                    A.mid[h][kh][j] = b;
+                    A.lo[h][kh][j] = c3;
+                } else {
+                    A.lo[h][kh][j] = b;
+                }
             }
     }
 }
@@ -337,7 +354,7 @@ __device__ __forceinline__ void dft16_ops(Dft16Ops& A) {
 // One radix-16 Stockham stage (same indexing as stage<>), butterflies taken
 // 16 at a time per wavefront.  Results are held in registers until every
 // wavefront has read its inputs (in place, like stage<>).
-template <int N, int NFFT, int NS, bool INV>
+template <int N, int NFFT, int NS, bool INV, bool S3>
 __device__ __forceinline__ void stage16(float2* buf, const float2* tw, const Dft16Ops& A) {
     constexpr int R = 16, Q = N / R, NB = NFFT * Q, G = NB / 16, NW = kBlock / 64, GPW = (G + NW - 1) / NW;
     constexpr int STEP = N / (R * NS);
@@ -351,15 +368,22 @@ __device__ __forceinline__ void stage16(float2* buf, const float2* tw, const Dft
         if (G % NW == 0 || grp < G) {              // uniform per wavefront
             const int bb = grp * 16 + c;
             const int f = bb / Q, b = bb - f * Q, k = b & (NS - 1);
-            v4bf bhi[2], blo[2];
+            v4bf bhi[2], bmid[2], blo[2];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = 4 * g + j;
                 float2 x = buf[pad(f * N + b + r * Q)];
                 if (NS > 1) x = cmul(x, twiddle<N, INV>(tw, r * k * STEP));
-                short h0, l0, h1, l1;
-                split_bf16(x.x, h0, l0);
-                split_bf16(x.y, h1, l1);
+                short h0, m0 = 0, l0, h1, m1 = 0, l1;
+                if (S3) {
+                    split3_bf16(x.x, h0, m0, l0);
+                    split3_bf16(x.y, h1, m1, l1);
+                    bmid[0][j] = m0;
+                    bmid[1][j] = m1;
+                } else {
+                    split_bf16(x.x, h0, l0);
+                    split_bf16(x.y, h1, l1);
+                }
                 bhi[0][j] = h0; blo[0][j] = l0;
                 bhi[1][j] = h1; blo[1][j] = l1;
             }
@@ -371,6 +395,14 @@ __device__ __forceinline__ void stage16(float2* buf, const float2* tw, const Dft
                 for (int kh = 0; kh < 2; ++kh) {
                     acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.lo[h][kh], bhi[kh], acc[h], 0, 0, 0);
                     acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.hi[h][kh], blo[kh], acc[h], 0, 0, 0);
+                    if (S3) acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.mid[h][kh], bmid[kh], acc[h], 0, 0, 0);
+                }
+                if (S3) {
+#pragma unroll
+                    for (int kh = 0; kh < 2; ++kh) {
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.mid[h][kh], bhi[kh], acc[h], 0, 0, 0);
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A.hi[h][kh], bmid[kh], acc[h], 0, 0, 0);
+                    }
                 }
 #pragma unroll
                 for (int kh = 0; kh < 2; ++kh)
@@ -393,38 +425,39 @@ __device__ __forceinline__ void stage16(float2* buf, const float2* tw, const Dft
     lds_barrier();
 }
 
-template <int N, int NFFT, int NS, bool INV>
+template <int N, int NFFT, int NS, bool INV, bool S3>
 __device__ __forceinline__ void stages16_from(float2* buf, const float2* tw, const Dft16Ops& A) {
     if constexpr (NS < N) {
-        stage16<N, NFFT, NS, INV>(buf, tw, A);
-        stages16_from<N, NFFT, NS * 16, INV>(buf, tw, A);
+        stage16<N, NFFT, NS, INV, S3>(buf, tw, A);
+        stages16_from<N, NFFT, NS * 16, INV, S3>(buf, tw, A);
     }
 }
 
 // NFFT transforms (as lds_fft) with radix-16 matrix-core stages after one
 // leading FP32 stage of radix 2, 4 or 8 for the leftover factor.
-template <int N, int NFFT, bool INV>
+template <int N, int NFFT, bool INV, bool S3>
 __device__ __forceinline__ void lds_fft_mfma(float2* buf, const float2* tw) {
     constexpr int L = ilog2c(N), REM = L % 4;
     Dft16Ops A;
-    dft16_ops<INV>(A);
+    dft16_ops<INV, S3>(A);
     if constexpr (REM == 0) {
-        stages16_from<N, NFFT, 1, INV>(buf, tw, A);
+        stages16_from<N, NFFT, 1, INV, S3>(buf, tw, A);
     } else {
         constexpr int R0 = 1 << REM;
         stage<N, NFFT, R0, 1, INV>(buf, tw);
-        stages16_from<N, NFFT, R0, INV>(buf, tw, A);
+        stages16_from<N, NFFT, R0, INV, S3>(buf, tw, A);
     }
 }
 
-// RADIX 4 or 8: FP32 Stockham stages; RADIX 16: the split-BF16 matrix-core
-// variant above.
+// Plan radix codes: 4 or 8 = FP32 Stockham stages; kRadixBf16x3 (16) and
+// kRadixBf16x6 (17) = the split-BF16 matrix-core radix-16 variants above.
+constexpr int kRadixBf16x3 = 16, kRadixBf16x6 = 17;
 template <int N, int NFFT, int RADIX, bool INV, int BT = fft_threads<N, RADIX>()>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
     constexpr int LR = ilog2c(RADIX), L = ilog2c(N), REM = L % LR;
-    if constexpr (RADIX == 16) {
+    if constexpr (RADIX == kRadixBf16x3 || RADIX == kRadixBf16x6) {
         static_assert(BT == kBlock, "the matrix-core stages run 256-thread blocks");
-        lds_fft_mfma<N, NFFT, INV>(buf, tw);
+        lds_fft_mfma<N, NFFT, INV, RADIX == kRadixBf16x6>(buf, tw);
     } else if constexpr (REM == 0) {
         stages_from<N, NFFT, RADIX, 1, INV, BT>(buf, tw);
     } else {

@@ -7,7 +7,8 @@
 // the C2R pass stages the potential row pair and both force rows of a pair in
 // one batch (three transforms), which frees the prefetch registers before the
 // transform starts.  Plan field `it` (rp = cb = 1); compiled for the default
-// radix at N = 1024 (radix 4), 2048 and 4096 (radix 8), IT in {2, 4}.
+// radix at N = 1024 (radix 4), 2048 and 4096 (radix 8), IT in {2, 4}, and for
+// the split-BF16 matrix-core radix codes 16 and 17 at N = 1024..4096, IT = 2.
 #include "pm_fft_core.h"
 
 namespace pmk {
@@ -288,6 +289,8 @@ hipError_t pipe_pass(int pass, int n, const FieldDesc* d, int count, const float
     if (n == NV && p.radix == RV && p.it == ITV) return launch_pass<NV, RV, ITV>(pass, d, count, tw, force, s);
     PM_PIPE(256, 4, 2) PM_PIPE(512, 4, 2) PM_PIPE(1024, 4, 2) PM_PIPE(1024, 4, 4) PM_PIPE(2048, 8, 2)
     PM_PIPE(2048, 8, 4) PM_PIPE(4096, 8, 2) PM_PIPE(4096, 8, 4)
+    PM_PIPE(1024, 16, 2) PM_PIPE(2048, 16, 2) PM_PIPE(4096, 16, 2)
+    PM_PIPE(1024, 17, 2) PM_PIPE(2048, 17, 2) PM_PIPE(4096, 17, 2)
 #undef PM_PIPE
     return hipErrorInvalidValue;
 }
@@ -296,6 +299,7 @@ hipError_t pipe_pass(int pass, int n, const FieldDesc* d, int count, const float
 
 bool pipe_plan_compiled(int n, int radix, int it) {
     if ((n == 256 || n == 512) && radix == 4) return it == 2;
+    if (radix == 16 || radix == 17) return it == 2 && (n == 1024 || n == 2048 || n == 4096);
     if (it != 2 && it != 4) return false;
     return (n == 1024 && radix == 4) || ((n == 2048 || n == 4096) && radix == 8);
 }

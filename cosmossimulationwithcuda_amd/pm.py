@@ -18,7 +18,7 @@ PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_EBUSY, PM_ECANCELLED, PM_ENODEV = ra
 NGP, CIC = 0, 1
 WRAP_KEEP_VELOCITY, WRAP_REFERENCE = 0, 1
 FORCE_SPECTRAL, FORCE_STENCIL = 0, 1
-FFT_FP32, FFT_BF16X3 = 0, 1
+FFT_FP32, FFT_BF16X3, FFT_BF16X6 = 0, 1, 2
 LATTICE_MASS, LATTICE_POTENTIAL, LATTICE_FORCE, LATTICE_TRANSFER = range(4)
 STAGE_DEPOSIT, STAGE_FIELD, STAGE_PUSH = range(3)
 TUNE_FFT, TUNE_PARTICLES = 1, 2

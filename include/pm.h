@@ -42,10 +42,11 @@ enum { PM_WRAP_KEEP_VELOCITY = 0, PM_WRAP_REFERENCE = 1 };
  * the reference's stencil (cuh:174-229) on every cell */
 enum { PM_FORCE_SPECTRAL = 0, PM_FORCE_STENCIL = 1 };
 /* FFT arithmetic: FP32 Stockham stages, or radix-16 stages on the matrix
- * cores in split BF16 (each FP32 operand as bf16 hi + lo, products
- * hi*hi + hi*lo + lo*hi accumulated in FP32; 256 <= n_lattice <= 8192).  The
- * spectra and diagnostics stay FP64 in both. */
-enum { PM_FFT_FP32 = 0, PM_FFT_BF16X3 = 1 };
+ * cores in split BF16 with FP32 accumulation (256 <= n_lattice <= 8192):
+ * BF16X3 splits each FP32 operand as bf16 hi + lo and forms hi*hi + hi*lo +
+ * lo*hi; BF16X6 splits it as hi + mid + lo and forms hi*hi + hi*mid + mid*hi
+ * + hi*lo + lo*hi + mid*mid.  The spectra and diagnostics stay FP64. */
+enum { PM_FFT_FP32 = 0, PM_FFT_BF16X3 = 1, PM_FFT_BF16X6 = 2 };
 /* lattices readable with pm_read_lattice */
 enum {
     PM_LATTICE_MASS = 0,      /* int64 fixed point, N*N, index i + j*N       */

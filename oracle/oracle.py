@@ -53,6 +53,12 @@ def lib():
         L.orc_step64.argtypes = [_c.c_int, _c.c_int, _c.c_double, _c.c_double, _c.c_int,
                                  _f64p, _f64p, _c.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p,
                                  _f64p, _f64p, _f64p, _f64p]
+        L.cpu32_create.argtypes = [_c.c_int, _c.c_int, _c.c_float, _c.c_float, _c.c_int, _c.c_int, _f64p, _f64p]
+        L.cpu32_create.restype = _c.c_void_p
+        L.cpu32_destroy.argtypes = [_c.c_void_p]
+        L.cpu32_step.argtypes = [_c.c_void_p, _c.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _c.c_int]
+        L.cpu32_potential.argtypes = [_c.c_void_p]
+        L.cpu32_potential.restype = _c.POINTER(_c.c_float)
         L.orc_num_threads.restype = _c.c_int
         L.orc_set_threads.argtypes = [_c.c_int]
         _lib = L
@@ -212,3 +218,30 @@ class Step64:
             L.orc_step64(self.n, self.mode, self.dt, self.g, self.wrap_mode, self.T[0], self.T[1],
                          len(self.x), self.x, self.y, self.vx, self.vy, self.m,
                          self.rho, self.phi, self.fx, self.fy)
+
+
+class Cpu32Stepper:
+    """The per-step algorithm in FP32 on the host cores (pm_cpu32.c): int64
+    deposit into per-thread lattices allocated once, FP32 FFT convolution,
+    the reference's FP32 stencil and push.  bench.py times it as the CPU
+    baseline."""
+
+    def __init__(self, n, mode, dt, g, g0, eps, wrap_mode, shift, x, y, vx, vy, m):
+        self.n = n
+        T = tuple(_c64(a) for a in transfer(n, mode, g0, eps))
+        self.x, self.y, self.vx, self.vy, self.m = (_c32(a).copy() for a in (x, y, vx, vy, m))
+        self._h = lib().cpu32_create(n, mode, float(dt), float(g), wrap_mode, shift, T[0], T[1])
+        if not self._h:
+            raise MemoryError("cpu32_create failed")
+
+    def step(self, k=1):
+        lib().cpu32_step(self._h, len(self.x), self.x, self.y, self.vx, self.vy, self.m, k)
+
+    def potential(self):
+        p = lib().cpu32_potential(self._h)
+        return np.ctypeslib.as_array(p, shape=(self.n * self.n,)).reshape(self.n, self.n).copy()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cpu32_destroy(self._h)
+            self._h = None

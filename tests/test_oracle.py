@@ -274,3 +274,26 @@ def test_mixed_step_tracks_fp64():
     b = orc.Step64(n, 1, 0.01, 1.0, 1.0, 1e-3, 0, x, y, z, z, m)
     a.step(5); b.step(5)
     assert np.abs(a.x - b.x).max() < 1e-3 and np.abs(a.y - b.y).max() < 1e-3
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cpu32_step_matches_oracle(mode):
+    """The FP32 CPU step (the CPU baseline): its potential is the oracle's FP64
+    potential of the same int64 deposit within 1e-5 relative L2 (FP32 FFT
+    rounding), and its trajectory tracks the FP64-field step."""
+    n, npart, shift = 64, 4000, 40
+    rng = np.random.default_rng(11 + mode)
+    x = rng.uniform(16, 48, npart).astype(np.float32)
+    y = rng.uniform(16, 48, npart).astype(np.float32)
+    z = np.zeros(npart, np.float32)
+    m = rng.lognormal(0.0, 0.5, npart).astype(np.float32)
+    c = orc.Cpu32Stepper(n, mode, 0.01, 1.0, 1.0, 1e-3, 0, shift, x, y, z, z, m)
+    c.step(1)
+    phi64 = orc.potential(n, orc.transfer(n, mode, 1.0, 1e-3), shift, orc.deposit(n, mode, shift, x, y, m))
+    phi32 = c.potential().astype(np.float64)
+    assert np.linalg.norm(phi32 - phi64) / np.linalg.norm(phi64) < 1e-5
+    a = orc.MixedStepper(n, mode, 0.01, 1.0, 1.0, 1e-3, 0, shift, x, y, z, z, m)
+    a.step(1)
+    c.step(4)
+    a.step(4)
+    assert np.abs(c.x - a.x).max() < 1e-3 and np.abs(c.y - a.y).max() < 1e-3

@@ -327,7 +327,7 @@ def serving_bench(nreq, seed, modes=("plain", "batched"), arrivals=("saturated",
     out = {"workload": f"ensemble_{nreq}req_poisson{SERVING['rate_per_s']:g}_{SERVING['steps']}steps_cic",
            "requests": nreq, "steps_per_request": SERVING["steps"],
            "lattices": list(SERVING["lattices"]), "particles_loguniform": list(SERVING["particles"]),
-           "budget_GB": SERVING["budget_bytes"] / 2 ** 30}
+           "budget_GB": SERVING["budget_bytes"] / 2 ** 30, "use_graphs": use_graphs}
     # One untimed pass over the first 32 requests in each server mode: the
     # first server run of a process pays one-time costs (code-object loading,
     # first device allocations, graph instantiation; ~0.7 s measured) that

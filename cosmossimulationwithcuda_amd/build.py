@@ -30,7 +30,7 @@ FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["pm_api.cpp", "pm_serve.cpp", "pm_binning.hip", "pm_fft.hip", "pm_fft_pipe.hip", "pm_fft_wave.hip", "pm_frame.hip", "pm_particles.hip", "pm_spectra.hip", "pm_util.hip"]
+SOURCES = ["pm_api.cpp", "pm_serve.cpp", "pm_binning.hip", "pm_fft.hip", "pm_fft_pipe.hip", "pm_fft_wave.hip", "pm_frame.hip", "pm_particles.hip", "pm_round.hip", "pm_spectra.hip", "pm_util.hip"]
 
 
 def _compile(src):

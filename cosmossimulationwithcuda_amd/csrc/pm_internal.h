@@ -218,6 +218,39 @@ struct PartDesc {
 hipError_t push_deposit_direct(int mode, const PartDesc* d, const int* block_start, int count, int total_blocks,
                                hipStream_t s);
 int direct_blocks(int64_t np);
+// ----- persistent round kernel (pm_round.hip) ------------------------------------
+// One batched member: field and particle descriptors per step parity (as the
+// per-pass batched launches use them; fd[p].rho_zero set), its twiddles,
+// lattice, deposit mode and items per step for the four passes (R2C, column,
+// C2R, push).
+struct RoundInst {
+    FieldDesc fd[2];
+    PartDesc pd[2];
+    const float2* tw;
+    int n, mode;
+    int items[4];
+};
+// A run of queue items: pass `phase` of member `inst` in step `step`,
+// starting at item `start`; segments are in queue order (step, pass, member).
+struct RoundSeg {
+    int start, inst, phase, step;
+};
+inline size_t round_ctl_bytes(int ninst) { return ((size_t)(2 + 4 * ninst) * 4 + 15) / 16 * 16; }
+// lattices and plans the kernel runs: the default FP32 plans at N = 256..2048
+bool round_supported(int n, const FftPlan& p);
+void round_items(int n, int64_t np, int items[4]);
+size_t round_lds_bytes(int max_n);
+// blocks of the kernel co-resident on the device at this LDS size (sets the
+// kernel's dynamic LDS limit; call outside stream capture)
+int round_max_blocks(int max_n, size_t lds_bytes);
+// wall-clock ticks of the device's constant-rate counter in `seconds`
+long long round_wait_ticks(double seconds);
+// steps x passes x members work queue; ctl: round_ctl_bytes(ninst) of device
+// memory (zeroed here, by a fill kernel), ctl[1] != 0 afterwards if a
+// dependency wait exceeded wait_ticks
+hipError_t round_persistent(const RoundInst* inst, const RoundSeg* seg, int nseg, int total, int* ctl, int ninst,
+                            int max_n, size_t lds_bytes, int blocks, long long wait_ticks, hipStream_t s);
+
 struct FillDesc {
     void* p;
     size_t bytes;   // multiple of 16

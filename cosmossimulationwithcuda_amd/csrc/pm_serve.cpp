@@ -47,6 +47,10 @@ double now_s() { return std::chrono::duration<double>(Clock::now().time_since_ep
 // separate admission and read-back threads: 2 lanes 4.79-4.83 s, 3: 4.51,
 // 4: 4.45-4.46, 5: 4.74-4.76, 6: 4.56-4.57.
 constexpr int kLanes = 4;
+// Limit of one dependency wait of the persistent round kernel: a safety net,
+// far above any real wait (one pass of one member, microseconds).
+constexpr double kRoundWaitSeconds = 2.0;
+constexpr int kRoundErrSlots = 4;      // more than the rounds in flight at once
 
 int64_t capacity_class(int64_t n) {
     int64_t c = 1024;
@@ -194,6 +198,13 @@ struct BatchPlan {
     size_t fill_off[2] = {0, 0};
     size_t max_fill = 0;
     hipGraphExec_t graph = nullptr;
+    // persistent round kernel over the members it runs (lattices up to 1024,
+    // default plans); the others keep the per-pass launches
+    std::vector<int> round_idx;
+    size_t round_inst_off = 0, round_seg_off = 0, round_ctl_off = 0;
+    int round_nseg = 0, round_total = 0, round_max_n = 0, round_blocks = 0;
+    size_t round_lds = 0;
+    long long round_wait_ticks = 0;
 };
 
 using SpectraKey = std::tuple<int, int, double, double>;   // lattice, mode, G(0), eps
@@ -212,6 +223,7 @@ struct InFlight {
     std::vector<Request*> completing;    // reach their step count (or failed) with this round
     hipEvent_t ev[1 + kLanes] = {};
     std::vector<hipGraphExec_t> retired;
+    int round_err_slot = -1;             // pinned word holding the persistent round's error flag
 };
 
 }  // namespace
@@ -258,6 +270,8 @@ struct pm_server {
     std::deque<InFlight> inflight;                   // launched rounds, oldest first (at most two)
     std::vector<hipEvent_t> event_pool;
     hipEvent_t batch_done = nullptr;                 // re-recorded after every batched launch
+    int* round_err_host = nullptr;                   // pinned, kRoundErrSlots words
+    int round_err_next = 0;
     double lane_load[kLanes] = {};
     double t_last_complete = 0;
     // streams (created once, default priority)
@@ -590,7 +604,8 @@ bool ensure_scratch(pm_server* s, size_t bytes) {
 // to `retired` and destroyed when the round being launched has completed.
 // The descriptor copy is ordered after the batch stream's earlier rounds, so
 // rewriting the device area does not disturb them.
-pm_status build_plan(pm_server* s, const std::vector<Request*>& members, std::vector<hipGraphExec_t>* retired) {
+pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int steps,
+                     std::vector<hipGraphExec_t>* retired) {
     BatchPlan& P = s->plan;
     if (P.graph) { retired->push_back(P.graph); P.graph = nullptr; }
     P.members = members;
@@ -598,8 +613,15 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, std::ve
     for (Request* r : members) P.signature.push_back(r->id);
     P.ngroups.clear();
     P.mgroups.clear();
+    P.round_idx.clear();
+    P.round_max_n = 0;
     for (int i = 0; i < (int)members.size(); ++i) {
         const pm_sim* h = members[i]->sim;
+        if (s->cfg.persistent && h->cfg.n_lattice <= 1024 && pmk::round_supported(h->cfg.n_lattice, h->plan)) {
+            P.round_idx.push_back(i);
+            P.round_max_n = std::max(P.round_max_n, h->cfg.n_lattice);
+            continue;
+        }
         auto ng = std::find_if(P.ngroups.begin(), P.ngroups.end(), [&](const BatchPlan::NGroup& g) {
             return g.n == h->cfg.n_lattice && g.plan.rp == h->plan.rp && g.plan.cb == h->plan.cb &&
                    g.plan.radix == h->plan.radix && g.plan.it == h->plan.it;
@@ -621,6 +643,19 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, std::ve
         g.bstart_off = take(sizeof(int) * (g.idx.size() + 1));
     }
     for (int p = 0; p < 2; ++p) P.fill_off[p] = take(sizeof(pmk::FillDesc) * members.size());
+    const int nr = (int)P.round_idx.size();
+    if (nr > 0) {
+        P.round_inst_off = take(sizeof(pmk::RoundInst) * nr);
+        P.round_seg_off = take(sizeof(pmk::RoundSeg) * (size_t)steps * 4 * nr);
+        P.round_ctl_off = take(pmk::round_ctl_bytes(nr));
+        P.round_lds = pmk::round_lds_bytes(P.round_max_n);
+        P.round_blocks = pmk::round_max_blocks(P.round_max_n, P.round_lds);
+        P.round_wait_ticks = pmk::round_wait_ticks(kRoundWaitSeconds);
+        if (P.round_blocks <= 0) {
+            s->batch_err = "persistent round kernel: no occupancy";
+            return PM_EDEVICE;
+        }
+    }
     if (!ensure_scratch(s, off)) {
         s->batch_err = "scratch allocation failed";
         return PM_ENOMEM;
@@ -681,6 +716,46 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, std::ve
             P.max_fill = std::max(P.max_fill, b);
         }
     }
+    if (nr > 0) {
+        auto* ri = reinterpret_cast<pmk::RoundInst*>(hb + P.round_inst_off);
+        auto* rs = reinterpret_cast<pmk::RoundSeg*>(hb + P.round_seg_off);
+        for (int k = 0; k < nr; ++k) {
+            const pm_sim* h = members[P.round_idx[k]]->sim;
+            pmk::RoundInst& I = ri[k];
+            for (int p = 0; p < 2; ++p) {
+                I.fd[p] = field_desc(h, h->rho_cur ^ p);
+                I.fd[p].rho_zero = h->rho_buf[h->rho_cur ^ p ^ 1];
+                pmk::PartDesc& q = I.pd[p];
+                q.x = h->cur.x;
+                q.y = h->cur.y;
+                q.vx = h->cur.vx;
+                q.vy = h->cur.vy;
+                q.m = h->cur.m;
+                q.force = h->force;
+                q.rho_next = reinterpret_cast<unsigned long long*>(h->rho_buf[h->rho_cur ^ p ^ 1]);
+                q.np = h->np;
+                q.n = h->cfg.n_lattice;
+                q.dt = h->cfg.dt;
+                q.scale = std::ldexp(1.0f, h->shift);
+                q.wrap_mode = h->cfg.wrap_mode;
+            }
+            I.tw = h->tw;
+            I.n = h->cfg.n_lattice;
+            I.mode = h->cfg.mode;
+            pmk::round_items(I.n, h->np, I.items);
+        }
+        // queue order: step, pass, member (empty runs left out)
+        int nseg = 0, total = 0;
+        for (int st = 0; st < steps; ++st)
+            for (int ph = 0; ph < 4; ++ph)
+                for (int k = 0; k < nr; ++k) {
+                    if (ri[k].items[ph] <= 0) continue;
+                    rs[nseg++] = pmk::RoundSeg{total, k, ph, st};
+                    total += ri[k].items[ph];
+                }
+        P.round_nseg = nseg;
+        P.round_total = total;
+    }
     if (hipMemcpyAsync(s->dscratch, hb, off, hipMemcpyHostToDevice, s->batch_stream) != hipSuccess ||
         hipEventRecord(s->hscratch_done[hi], s->batch_stream) != hipSuccess) {
         s->batch_err = "descriptor upload failed";
@@ -692,6 +767,13 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, std::ve
 pm_status launch_batched_steps(pm_server* s, int steps) {
     BatchPlan& P = s->plan;
     char* db = static_cast<char*>(s->dscratch);
+    if (!P.round_idx.empty() &&
+        pmk::round_persistent(reinterpret_cast<const pmk::RoundInst*>(db + P.round_inst_off),
+                              reinterpret_cast<const pmk::RoundSeg*>(db + P.round_seg_off), P.round_nseg,
+                              P.round_total, reinterpret_cast<int*>(db + P.round_ctl_off), (int)P.round_idx.size(),
+                              P.round_max_n, P.round_lds, std::min(P.round_blocks, P.round_total), P.round_wait_ticks,
+                              s->batch_stream) != hipSuccess)
+        return PM_EDEVICE;
     for (int st = 0; st < steps; ++st) {
         const int p = st & 1;
         for (auto& g : P.ngroups) {
@@ -732,7 +814,7 @@ pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, 
     for (Request* r : members) sig.push_back(r->id);
     pm_status st = PM_OK;
     if (sig != s->plan.signature || s->plan.members.empty()) {
-        if ((st = build_plan(s, members, retired)) != PM_OK) return st;
+        if ((st = build_plan(s, members, steps, retired)) != PM_OK) return st;
     }
     if (s->cfg.use_graphs) {
         if (!s->plan.graph) {
@@ -856,6 +938,15 @@ void launch_round(pm_server* s) {
         if (run_batched_round(s, batched, R, &f.retired) != PM_OK) {
             for (Request* r : batched) { r->msg = s->batch_err; bad.push_back(r); }
             s->plan.signature.clear();
+        } else if (!s->plan.round_idx.empty()) {
+            // the round kernel's error flag, copied behind it (the next round clears it)
+            const int slot = s->round_err_next;
+            s->round_err_next = (slot + 1) % kRoundErrSlots;
+            s->round_err_host[slot] = 0;
+            const int* flag = reinterpret_cast<const int*>(static_cast<char*>(s->dscratch) + s->plan.round_ctl_off) + 1;
+            if (hipMemcpyAsync(&s->round_err_host[slot], flag, sizeof(int), hipMemcpyDeviceToHost, s->batch_stream) ==
+                hipSuccess)
+                f.round_err_slot = slot;
         }
         batch_used = true;
         (void)hipEventRecord(s->batch_done, s->batch_stream);
@@ -920,11 +1011,14 @@ void complete_round(pm_server* s) {
         }
     const double t1 = now_s();
     for (hipGraphExec_t g : f.retired) (void)hipGraphExecDestroy(g);
+    const bool round_err = f.round_err_slot >= 0 && s->round_err_host[f.round_err_slot] != 0;
+    if (round_err) err[0] = true;
     for (const InFlight::Advance& a : f.advance) {
         Request* r = a.r;
         if (err[a.stream] && !r->failed) {
             r->failed = true;
-            r->msg = "device error";
+            r->msg = a.stream == 0 && round_err ? "persistent round kernel: a dependency wait hit its spin limit"
+                                                : "device error";
             if (std::find(f.completing.begin(), f.completing.end(), r) == f.completing.end()) {
                 // still in the active list: drop it there, finish it with this round
                 s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
@@ -1030,6 +1124,7 @@ static void free_server(pm_server* s) {
     }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     if (s->batch_done) (void)hipEventDestroy(s->batch_done);
+    if (s->round_err_host) (void)hipHostFree(s->round_err_host);
     if (s->dscratch) (void)hipFree(s->dscratch);
     for (int i = 0; i < 2; ++i) {
         if (s->hscratch[i]) (void)hipHostFree(s->hscratch[i]);
@@ -1046,6 +1141,7 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     if (!cfg || !out) return PM_EINVAL;
     *out = nullptr;
     if (cfg->hbm_budget_bytes <= 0 || cfg->max_active <= 0 || cfg->round_steps <= 0 || cfg->pinned_staging_mib < 0 ||
+        cfg->persistent != 0 ||   // the persistent round kernel is disabled (it did not complete on the device)
         (cfg->mode != PM_SERVE_PLAIN && cfg->mode != PM_SERVE_BATCHED))
         return PM_EINVAL;
     if (pm_device_count() <= cfg->device || cfg->device < 0) return PM_ENODEV;
@@ -1061,6 +1157,7 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
         ok = ok && hipStreamCreateWithFlags(q, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& q : s->lanes) ok = ok && hipStreamCreateWithFlags(&q, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&s->batch_done, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipHostMalloc(reinterpret_cast<void**>(&s->round_err_host), sizeof(int) * kRoundErrSlots) == hipSuccess;
     if (!ok) {
         free_server(s);
         return PM_EDEVICE;

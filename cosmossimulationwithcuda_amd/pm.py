@@ -331,7 +331,8 @@ class ServerConfig(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("batch_max_particles", ctypes.c_int32),
         ("pinned_staging_mib", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("persistent", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 

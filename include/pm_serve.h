@@ -51,7 +51,13 @@ typedef struct pm_server_config {
                                   * holds submitted arrays until their results are collected:
                                   * uploads and read-backs from it are asynchronous DMA.  A request
                                   * that finds no room uses pageable memory.  0 = none. */
-    int32_t reserved[6];
+    int32_t persistent;          /* BATCHED: run each round's steps of the members with lattices up
+                                  * to 1024 (default FFT plans) in one persistent kernel that pulls
+                                  * row, column and particle items from a device-side queue.
+                                  * Disabled in this release (the kernel did not complete its first
+                                  * rounds on the device; cause not found): must be 0, any other
+                                  * value is PM_EINVAL. */
+    int32_t reserved[5];
 } pm_server_config;
 
 typedef struct pm_server_stats {

@@ -28,6 +28,13 @@ def run_alone(cfg, arrays, steps):
         return s.read()
 
 
+# The persistent round kernel (server config `persistent`) hung on the device
+# in the eager mix case, twice: no request of the round completed within the
+# 120 s wait and the server's destroy then blocked on the unfinished round.
+# The library rejects persistent != 0 until the cause is found; its cases
+# stay here, skipped.
+PERSISTENT_OFF = "persistent round kernel disabled: it did not complete on the device (cause not found)"
+
 MIX = [  # (lattice, particles, mode, steps, extra config)
     (256, 20000, pm.CIC, 37, {}),
     (256, 5000, pm.NGP, 24, {"wrap_mode": pm.WRAP_REFERENCE}),
@@ -45,19 +52,22 @@ def mix():
     return reqs, want
 
 
-@pytest.mark.parametrize("mode,graphs,batch_max,pinned_mib", [
-    (pm.SERVE_BATCHED, 1, 512, 1024),
-    (pm.SERVE_BATCHED, 0, 512, 1024),
-    (pm.SERVE_BATCHED, 1, 0, 1024),       # no batched launches: every request on its own stream
-    (pm.SERVE_PLAIN, 0, 512, 1024),
-    (pm.SERVE_BATCHED, 1, 512, 0),        # request arrays in pageable memory
-    (pm.SERVE_BATCHED, 1, 512, 2),        # 2 MiB of pinned staging: some requests pinned, the rest pageable
+@pytest.mark.parametrize("mode,graphs,batch_max,pinned_mib,persistent", [
+    (pm.SERVE_BATCHED, 1, 512, 1024, 0),
+    (pm.SERVE_BATCHED, 0, 512, 1024, 0),
+    (pm.SERVE_BATCHED, 1, 0, 1024, 0),       # no batched launches: every request on its own stream
+    (pm.SERVE_PLAIN, 0, 512, 1024, 0),
+    (pm.SERVE_BATCHED, 1, 512, 0, 0),        # request arrays in pageable memory
+    (pm.SERVE_BATCHED, 1, 512, 2, 0),        # 2 MiB of pinned staging: some requests pinned, the rest pageable
+    pytest.param(pm.SERVE_BATCHED, 0, 512, 1024, 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+    pytest.param(pm.SERVE_BATCHED, 1, 512, 1024, 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
 ])
-def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max, pinned_mib):
+def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max, pinned_mib, persistent):
     reqs, want = mix
-    print(f"serving mode={mode} graphs={graphs} batch_max_lattice={batch_max} pinned={pinned_mib} MiB", flush=True)
+    print(f"serving mode={mode} graphs={graphs} batch_max_lattice={batch_max} pinned={pinned_mib} MiB "
+          f"persistent={persistent}", flush=True)
     with pm.Server(mode=mode, use_graphs=graphs, batch_max_lattice=batch_max, pinned_staging_mib=pinned_mib,
-                   hbm_budget_bytes=8 << 30) as srv:
+                   persistent=persistent, hbm_budget_bytes=8 << 30) as srv:
         ids = [srv.submit(c, *a, steps=st) for (c, a), (_, _, _, st, _) in zip(reqs, MIX)]
         got = [srv.wait(i, timeout_s=120) for i in ids]
         stats = srv.stats()
@@ -152,12 +162,17 @@ def c5():
     return reqs, cfgs, arrays, steps, want
 
 
-@pytest.mark.parametrize("arrivals", ["saturated", "stream"])
-def test_config5_ensemble_matches_alone_bitwise(c5, arrivals):
+@pytest.mark.parametrize("arrivals,persistent", [
+    ("saturated", 0), ("stream", 0),
+    pytest.param("saturated", 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+    pytest.param("stream", 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+])
+def test_config5_ensemble_matches_alone_bitwise(c5, arrivals, persistent):
     import time
     reqs, cfgs, arrays, steps, want = c5
-    print(f"config 5 subset: {len(reqs)} requests, {sum(r['n'] for r in reqs)} particles, {arrivals}", flush=True)
-    with pm.Server(mode=pm.SERVE_BATCHED, use_graphs=1) as srv:
+    print(f"config 5 subset: {len(reqs)} requests, {sum(r['n'] for r in reqs)} particles, {arrivals}, "
+          f"persistent={persistent}", flush=True)
+    with pm.Server(mode=pm.SERVE_BATCHED, use_graphs=1, persistent=persistent) as srv:
         ids = []
         for c, a in zip(cfgs, arrays):
             ids.append(srv.submit(c, *a, steps=steps))

@@ -179,6 +179,99 @@ def precision_report(seed):
     return out
 
 
+def field_report(sizes, reps, seed):
+    """The three FFT passes of the field solve (stage FIELD: row R2C, column
+    pass with the transfer function and the y derivative, row C2R with the
+    edge bands) per lattice size, one simulation, default plan: average
+    duration from HIP events and achieved GB/s of the algorithmic bytes
+    (the R2C pass of the stage API does not zero a next lattice)."""
+    from cosmossimulationwithcuda_amd import ics, pm
+    out = {}
+    for n in sizes:
+        np_ = 1_000_000
+        arrs = ics.uniform_square(np_, n // 4, 3 * n // 4, seed=seed)
+        cfg = pm.default_config(n_lattice=n, n_particles=np_, mode=pm.CIC, dt=0.001, gravity=1.0, green_origin=1.0,
+                                deconv_eps=1e-3)
+        with pm.Simulation(cfg) as s:
+            s.load(*arrs)
+            s.stage(pm.STAGE_DEPOSIT)
+            s.stage(pm.STAGE_FIELD)
+            s.profile(True)
+            s.profile_reset()
+            for _ in range(reps):
+                s.stage(pm.STAGE_FIELD)
+            s.sync()
+            prof = s.profile_read()
+            plan = {k: s.get_param(k) for k in ("fft_radix", "fft_it")}
+        nk = n // 2 + 1
+        row = {"plan": plan}
+        for k, gb in (("row_r2c", (8 * n * n + 8 * nk * n) / 1e9), ("col_conv", algorithmic_bytes("col_conv", n, 0) / 1e9),
+                      ("row_c2r", algorithmic_bytes("row_c2r", n, 0) / 1e9)):
+            ms, cnt = prof[k]
+            us = ms / cnt * 1e3
+            row[k] = {"avg_us": round(us, 2), "algorithmic_MB": round(gb * 1e3, 2), "GB_s": round(gb / (us * 1e-6), 1)}
+        row["vendor_fft_reference_us"] = vendor_fft_reference(n, reps)
+        out[str(n)] = row
+        print(f"# field {n}: {row}", file=sys.stderr, flush=True)
+    return out
+
+
+def vendor_fft_reference(n, reps):
+    """The ROCm FFT library (hipFFT over rocFFT) on the same device, for scale:
+    a batched 1D C2C of N/2 rows of length N (the work of one row pass: two
+    real rows per complex transform), and the plain 2D R2C and C2R of an N^2
+    lattice.  Average us per call, HIP events on the null stream; None if the
+    library is absent."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        fft = ctypes.CDLL("libhipfft.so")
+    except OSError:
+        return None
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    nn = n * n
+    a, b = vp(), vp()
+    assert hip.hipMalloc(ctypes.byref(a), sz(nn * 16)) == 0 and hip.hipMalloc(ctypes.byref(b), sz(nn * 16)) == 0
+    assert hip.hipMemset(a, 0, sz(nn * 16)) == 0 and hip.hipMemset(b, 0, sz(nn * 16)) == 0
+    ev0, ev1 = vp(), vp()
+    assert hip.hipEventCreate(ctypes.byref(ev0)) == 0 and hip.hipEventCreate(ctypes.byref(ev1)) == 0
+
+    def timed(call):
+        for _ in range(3):
+            assert call() == 0
+        hip.hipEventRecord(ev0, vp(0))
+        for _ in range(reps):
+            assert call() == 0
+        hip.hipEventRecord(ev1, vp(0))
+        hip.hipEventSynchronize(ev1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), ev0, ev1)
+        return round(ms.value / reps * 1e3, 2)
+
+    C2C, R2C, C2R, FWD = 0x29, 0x2a, 0x2c, -1
+    for f in (fft.hipfftExecC2C, fft.hipfftExecR2C, fft.hipfftExecC2R, fft.hipfftDestroy):
+        f.restype = ctypes.c_int
+    fft.hipfftExecC2C.argtypes = [vp, vp, vp, ctypes.c_int]
+    fft.hipfftExecR2C.argtypes = fft.hipfftExecC2R.argtypes = [vp, vp, vp]
+    fft.hipfftDestroy.argtypes = [vp]
+    out = {}
+    plans = [vp(), vp(), vp()]                   # hipfftHandle is a pointer on ROCm
+    length = (ctypes.c_int * 1)(n)
+    assert fft.hipfftPlanMany(ctypes.byref(plans[0]), 1, length, None, 1, n, None, 1, n, C2C, n // 2) == 0
+    out["c2c_1d_rows_half"] = timed(lambda: fft.hipfftExecC2C(plans[0], a, b, FWD))
+    assert fft.hipfftPlan2d(ctypes.byref(plans[1]), n, n, R2C) == 0
+    out["r2c_2d"] = timed(lambda: fft.hipfftExecR2C(plans[1], a, b))
+    assert fft.hipfftPlan2d(ctypes.byref(plans[2]), n, n, C2R) == 0
+    out["c2r_2d"] = timed(lambda: fft.hipfftExecC2R(plans[2], b, a))
+    for q in plans:
+        fft.hipfftDestroy(q)
+    hip.hipEventDestroy(ev0)
+    hip.hipEventDestroy(ev1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return out
+
+
 # Serving ensemble: independent requests arriving as a Poisson stream.
 SERVING = {"requests": 512, "rate_per_s": 40.0, "lattices": (256, 512, 1024, 2048), "particles": (1e4, 1e7),
            "steps": 200, "budget_bytes": 200 << 30, "dt": 1.0, "template": 10_000_000,
@@ -380,6 +473,9 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="only the 4096^2 Plummer-binary FP32 vs split-BF16 (bf16x3, bf16x6) FFT comparison "
                          "(error vs FP64 per step)")
+    ap.add_argument("--field-report", default="",
+                    help="comma-separated lattice sizes: time the FFT passes of the field solve at each and exit")
+    ap.add_argument("--field-reps", type=int, default=20)
     ap.add_argument("--serving-requests", type=int, default=SERVING["requests"])
     ap.add_argument("--serving-modes", default="plain,batched,naive,cpu")
     ap.add_argument("--serving-arrivals", default="saturated,poisson")
@@ -395,6 +491,10 @@ def main():
 
     if args.precision_report:
         print(json.dumps({"precision": precision_report(args.seed)}), flush=True)
+        return
+    if args.field_report:
+        print(json.dumps({"field": field_report([int(v) for v in args.field_report.split(",")], args.field_reps,
+                                                args.seed)}), flush=True)
         return
     if args.serving == "only":
         print(json.dumps({"serving": serving_bench(args.serving_requests, args.seed, args.serving_modes.split(","),

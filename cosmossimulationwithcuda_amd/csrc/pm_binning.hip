@@ -47,10 +47,10 @@ __device__ __forceinline__ int chunks_of(int count, int chunk, int kmax) {
 // old_start: the previous starts (may alias start); standing_cursor: start
 // the parked bin's cursor after the standing particles (fused re-sort, which
 // moves them separately).
-__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count, int nb, int chunk, int kmax,
-                                                          int add_parked, int standing_cursor,
-                                                          const int64_t* old_start, int64_t* start,
-                                                          int* __restrict__ chunk_start, int* __restrict__ cursor) {
+__device__ __forceinline__ void bin_scan_block(int* __restrict__ count, int nb, int chunk, int kmax,
+                                               int add_parked, int standing_cursor, const int64_t* old_start,
+                                               int64_t* start, int* __restrict__ chunk_start,
+                                               int* __restrict__ cursor) {
     __shared__ int64_t s_cnt[kScanBlock];
     __shared__ int s_chk[kScanBlock];
     __shared__ int s_standing;
@@ -89,6 +89,26 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count
         if (b < nb - 1) runk += chunks_of(count[b], chunk, kmax);
     }
     if (t == kScanBlock - 1) start[nb] = s_cnt[t];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int* __restrict__ count, int nb, int chunk, int kmax,
+                                                          int add_parked, int standing_cursor,
+                                                          const int64_t* old_start, int64_t* start,
+                                                          int* __restrict__ chunk_start, int* __restrict__ cursor) {
+    bin_scan_block(count, nb, chunk, kmax, add_parked, standing_cursor, old_start, start, chunk_start, cursor);
+}
+
+// scan_next of simulation blockIdx.x of a batched round.
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan_batched(const TiledDesc* __restrict__ descs) {
+    const TiledDesc& d = descs[blockIdx.x];
+    bin_scan_block(d.next_count, d.nbins, d.chunk, d.kmax, 1, 1, d.start, d.next_start, d.next_chunk_start,
+                   d.next_cursor);
+}
+
+// next_count = 0 for simulation blockIdx.y of a batched round.
+__global__ __launch_bounds__(kBlock) void k_zero_next_counts(const TiledDesc* __restrict__ descs) {
+    const TiledDesc& d = descs[blockIdx.y];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.nbins; i += gridDim.x * kBlock) d.next_count[i] = 0;
 }
 
 // Counting-sort scatter.  A block of kScatBlock threads takes kScatGroups
@@ -234,6 +254,19 @@ hipError_t scan_next(int n, BinState next, const int64_t* old_start, hipStream_t
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanBlock), 0, s, next.count, nb, next.chunk,
                        next.max_chunks_per_bin > 0 ? next.max_chunks_per_bin : (1 << 30), 1, 1, old_start, next.start,
                        next.chunk_start, next.cursor);
+    return hipGetLastError();
+}
+
+hipError_t scan_next_batched(const TiledDesc* d, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_scan_batched, dim3(count), dim3(kScanBlock), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t zero_next_counts_batched(const TiledDesc* d, int count, int max_nbins, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int bx = (max_nbins + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_zero_next_counts, dim3(bx < 64 ? bx : 64, count), dim3(kBlock), 0, s, d);
     return hipGetLastError();
 }
 

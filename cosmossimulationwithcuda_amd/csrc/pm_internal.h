@@ -251,6 +251,41 @@ long long round_wait_ticks(double seconds);
 hipError_t round_persistent(const RoundInst* inst, const RoundSeg* seg, int nseg, int total, int* ctl, int ninst,
                             int max_n, size_t lds_bytes, int blocks, long long wait_ticks, hipStream_t s);
 
+// One simulation of a batched serving round that keeps its particles sorted
+// by tile (more particles than the direct kernel suits): its step kernels,
+// the fused re-sort's counting, scan and re-sorting steps and the parked-range
+// copy run over arrays of these, one launch per group of equal lattice size,
+// tile size, deposit mode and wrap mode (pm_particles.hip, pm_binning.hip).
+struct TiledDesc {
+    ParticleArrays cur, alt;              // current order; target of the re-sort
+    const float2* force;
+    unsigned long long* rho_next[2];      // deposit target by step parity
+    const int64_t* start;                 // bins of the current order
+    const int* chunk_start;
+    int* next_count;                      // second bin set, filled by the counting step
+    int64_t* next_start;
+    int* next_chunk_start;
+    int* next_cursor;
+    unsigned long long* migrants;
+    int n, nbins, chunk, kmax;            // lattice, bins incl. the parked one, deposit chunk, chunks per bin cap
+    float dt, scale;                      // time step, 2^shift
+};
+// block_start[i]: first block of simulation i (exclusive prefix of its
+// max_chunks), block_start[count] = total_blocks.  p: parity of the step
+// (rho_next[p] is the deposit target).  count_bins: the counting step
+// (histogram of the new bins into next_count, zero on entry).
+hipError_t push_deposit_batched(int n, int tl, int mode, int wrap_mode, bool count_bins, bool pipelined,
+                                const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
+                                hipStream_t s);
+// The re-sorting step (cur -> alt in the order of the counted bins, with the
+// standing parked particles) of every simulation.
+hipError_t push_deposit_scatter_batched(int n, int tl, int mode, int wrap_mode, const TiledDesc* d,
+                                        const int* block_start, int count, int total_blocks, int p, hipStream_t s);
+// next_count = 0 (nbins entries) of every simulation.
+hipError_t zero_next_counts_batched(const TiledDesc* d, int count, int max_nbins, hipStream_t s);
+// scan_next of every simulation (next bins from next_count; old starts = start).
+hipError_t scan_next_batched(const TiledDesc* d, int count, hipStream_t s);
+
 struct FillDesc {
     void* p;
     size_t bytes;   // multiple of 16

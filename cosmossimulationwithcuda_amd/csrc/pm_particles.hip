@@ -129,12 +129,15 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit_bin(const float* __re
 // blocks per CU instead of three.
 template <int TL> constexpr int fused_block() { return TL <= 5 ? 512 : 1024; }
 
+// Chunk c of one simulation; the single-simulation kernel runs chunk
+// blockIdx.x, the batched one (k_push_deposit_batched) maps its blocks over
+// the chunks of several simulations.
 template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
-__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
+__device__ __forceinline__ void push_deposit_block(
     const float2* __restrict__ force, float* __restrict__ x, float* __restrict__ y, float* __restrict__ vx,
     float* __restrict__ vy, const float* __restrict__ m, int n, float dt, const int64_t* __restrict__ start,
-    const int* __restrict__ chunk_start, int chunk, float scale, unsigned long long* __restrict__ rho,
-    unsigned long long* __restrict__ migrants, int* __restrict__ count) {
+    const int* __restrict__ chunk_start, float scale, unsigned long long* __restrict__ rho,
+    unsigned long long* __restrict__ migrants, int* __restrict__ count, int c) {
     constexpr int W = TileGeom<TL>::kWin, E = TileGeom<TL>::kEdge, BLK = fused_block<TL>();
     __shared__ unsigned long long win[W * W];
     __shared__ float2 fwin[W * W];
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     const int tpr = n >> TL, nin = tpr * tpr;
     int b;
     int64_t s0, s1;
-    if (!chunk_of(chunk_start, start, nin, blockIdx.x, b, s0, s1)) return;
+    if (!chunk_of(chunk_start, start, nin, c, b, s0, s1)) return;
     const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
     for (int e = threadIdx.x; e < W * W; e += BLK) {
         win[e] = 0ull;
@@ -290,6 +293,42 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }
 
+template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
+__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
+    const float2* __restrict__ force, float* __restrict__ x, float* __restrict__ y, float* __restrict__ vx,
+    float* __restrict__ vy, const float* __restrict__ m, int n, float dt, const int64_t* __restrict__ start,
+    const int* __restrict__ chunk_start, float scale, unsigned long long* __restrict__ rho,
+    unsigned long long* __restrict__ migrants, int* __restrict__ count) {
+    push_deposit_block<TL, MODE, WRAP, COUNT, PIPE>(force, x, y, vx, vy, m, n, dt, start, chunk_start, scale, rho,
+                                                    migrants, count, blockIdx.x);
+}
+
+// Simulation of block blockIdx.x in a batched launch: the last i with
+// block_start[i] <= blockIdx.x (block_start[count] = total blocks).
+__device__ __forceinline__ int batch_member(const int* __restrict__ block_start, int count) {
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (block_start[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The step kernel over several simulations of one lattice size, deposit mode
+// and wrap mode (a batched serving round): the blocks of simulation i are
+// block_start[i] .. block_start[i + 1] - 1, one per possible chunk of it.
+// p: the step's parity (which mass lattice is the deposit target).
+template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
+__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_batched(const TiledDesc* __restrict__ descs,
+                                                                           const int* __restrict__ block_start,
+                                                                           int count, int p) {
+    const int i = batch_member(block_start, count);
+    const TiledDesc& d = descs[i];
+    push_deposit_block<TL, MODE, WRAP, COUNT, PIPE>(d.force, d.cur.x, d.cur.y, d.cur.vx, d.cur.vy, d.cur.m, d.n, d.dt,
+                                                    d.start, d.chunk_start, d.scale, d.rho_next[p], d.migrants,
+                                                    d.next_count, (int)blockIdx.x - block_start[i]);
+}
+
 // --------------------------------------------- fused step kernel with re-sort --
 // The re-sort fused into a step: the chunk's particles (read from `src` in the
 // order of the current chunk map) are pushed and deposited as above, then
@@ -300,11 +339,11 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
 // previous step, so `dst` ends in the order of this step's start positions,
 // one step stale, well inside the kHalo margin of the tile windows.
 template <int TL, int MODE, int WRAP>
-__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
-    const float2* __restrict__ force, ParticleArrays src, ParticleArrays dst, int n, float dt,
+__device__ __forceinline__ void push_deposit_scatter_block(
+    const float2* __restrict__ force, const ParticleArrays& src, const ParticleArrays& dst, int n, float dt,
     const int64_t* __restrict__ start, const int* __restrict__ chunk_start, const int64_t* __restrict__ next_start,
     int* __restrict__ next_cursor, float scale, unsigned long long* __restrict__ rho,
-    unsigned long long* __restrict__ migrants) {
+    unsigned long long* __restrict__ migrants, int c) {
     constexpr int W = TileGeom<TL>::kWin, E = TileGeom<TL>::kEdge, BLK = fused_block<TL>();
     __shared__ unsigned long long win[W * W];
     __shared__ float2 fwin[W * W];
@@ -315,7 +354,7 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
     int* lbase = lds + nin + 1;  // [nin + 1]
     int b;
     int64_t s0, s1;
-    if (!chunk_of(chunk_start, start, nin, blockIdx.x, b, s0, s1)) return;
+    if (!chunk_of(chunk_start, start, nin, c, b, s0, s1)) return;
     const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
     for (int e = threadIdx.x; e < W * W; e += BLK) {
         win[e] = 0ull;
@@ -438,6 +477,28 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }
 
+template <int TL, int MODE, int WRAP>
+__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
+    const float2* __restrict__ force, ParticleArrays src, ParticleArrays dst, int n, float dt,
+    const int64_t* __restrict__ start, const int* __restrict__ chunk_start, const int64_t* __restrict__ next_start,
+    int* __restrict__ next_cursor, float scale, unsigned long long* __restrict__ rho,
+    unsigned long long* __restrict__ migrants) {
+    push_deposit_scatter_block<TL, MODE, WRAP>(force, src, dst, n, dt, start, chunk_start, next_start, next_cursor,
+                                               scale, rho, migrants, blockIdx.x);
+}
+
+// The re-sorting step over several simulations (a batched serving round);
+// block mapping as k_push_deposit_batched.
+template <int TL, int MODE, int WRAP>
+__global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter_batched(
+    const TiledDesc* __restrict__ descs, const int* __restrict__ block_start, int count, int p) {
+    const int i = batch_member(block_start, count);
+    const TiledDesc& d = descs[i];
+    push_deposit_scatter_block<TL, MODE, WRAP>(d.force, d.cur, d.alt, d.n, d.dt, d.start, d.chunk_start,
+                                               d.next_start, d.next_cursor, d.scale, d.rho_next[p], d.migrants,
+                                               (int)blockIdx.x - block_start[i]);
+}
+
 // Particles standing in the parked bin (never visited by the step kernels)
 // move to the front of the parked range of `dst`; the scan started that
 // bin's cursor after them.
@@ -509,6 +570,84 @@ __global__ __launch_bounds__(kBlock) void k_push_deposit_direct(const PartDesc* 
     direct_push_group<MODE>(descs[lo], (int64_t)(blockIdx.x - block_start[lo]) * kBlock + threadIdx.x);
 }
 
+// Standing parked particles of simulation blockIdx.y of a batched round (as
+// k_copy_parked: from the parked range of cur to the front of alt's).
+__global__ __launch_bounds__(kBlock) void k_copy_parked_batched(const TiledDesc* __restrict__ descs) {
+    const TiledDesc& d = descs[blockIdx.y];
+    const int parked = d.nbins - 1;
+    const int64_t a = d.start[parked], cnt = d.start[parked + 1] - a, o = d.next_start[parked];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * kBlock) {
+        d.alt.x[o + i] = d.cur.x[a + i];
+        d.alt.y[o + i] = d.cur.y[a + i];
+        d.alt.vx[o + i] = d.cur.vx[a + i];
+        d.alt.vy[o + i] = d.cur.vy[a + i];
+        d.alt.m[o + i] = d.cur.m[a + i];
+        d.alt.id[o + i] = d.cur.id[a + i];
+    }
+}
+
+// Batched launchers: dynamic LDS as the single-simulation launchers (the bin
+// histogram of the counting step; the scatter's two bin tables).
+template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
+hipError_t push_deposit_batched_k(int n, const TiledDesc* d, const int* block_start, int count, int total_blocks,
+                                  int p, hipStream_t s) {
+    constexpr int W = TileGeom<TL>::kWin, BLK = fused_block<TL>();
+    const size_t lds = COUNT ? sizeof(int) * ((n >> TL) * (n >> TL) + 1) : 0;
+    if (COUNT && lds + 16 * W * W > 65536) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&k_push_deposit_batched<TL, MODE, WRAP, COUNT, PIPE>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_push_deposit_batched<TL, MODE, WRAP, COUNT, PIPE>), dim3(total_blocks), dim3(BLK), lds, s,
+                       d, block_start, count, p);
+    return hipGetLastError();
+}
+
+template <int TL, int MODE, int WRAP>
+hipError_t push_deposit_batched_t(int n, bool count_bins, bool pipelined, const TiledDesc* d, const int* block_start,
+                                  int count, int total_blocks, int p, hipStream_t s) {
+    if (count_bins)
+        return pipelined ? push_deposit_batched_k<TL, MODE, WRAP, true, true>(n, d, block_start, count, total_blocks, p, s)
+                         : push_deposit_batched_k<TL, MODE, WRAP, true, false>(n, d, block_start, count, total_blocks, p, s);
+    return pipelined ? push_deposit_batched_k<TL, MODE, WRAP, false, true>(n, d, block_start, count, total_blocks, p, s)
+                     : push_deposit_batched_k<TL, MODE, WRAP, false, false>(n, d, block_start, count, total_blocks, p, s);
+}
+
+template <int TL>
+hipError_t push_deposit_batched_mode(int n, int mode, int wrap_mode, bool count_bins, bool pipelined,
+                                     const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
+                                     hipStream_t s) {
+    if (mode == 0 && wrap_mode == 0)
+        return push_deposit_batched_t<TL, 0, 0>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+    if (mode == 0)
+        return push_deposit_batched_t<TL, 0, 1>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+    if (wrap_mode == 0)
+        return push_deposit_batched_t<TL, 1, 0>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+    return push_deposit_batched_t<TL, 1, 1>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+}
+
+template <int TL, int MODE, int WRAP>
+hipError_t scatter_batched_t(int n, const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
+                             hipStream_t s) {
+    const size_t lds = 2 * sizeof(int) * (size_t)(((n >> TL) * (n >> TL)) + 1);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_scatter_batched<TL, MODE, WRAP>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_push_deposit_scatter_batched<TL, MODE, WRAP>), dim3(total_blocks), dim3(fused_block<TL>()),
+                       lds, s, d, block_start, count, p);
+    return hipGetLastError();
+}
+
+template <int TL>
+hipError_t scatter_batched_mode(int n, int mode, int wrap, const TiledDesc* d, const int* block_start, int count,
+                                int total_blocks, int p, hipStream_t s) {
+    if (mode == 0 && wrap == 0) return scatter_batched_t<TL, 0, 0>(n, d, block_start, count, total_blocks, p, s);
+    if (mode == 0) return scatter_batched_t<TL, 0, 1>(n, d, block_start, count, total_blocks, p, s);
+    if (wrap == 0) return scatter_batched_t<TL, 1, 0>(n, d, block_start, count, total_blocks, p, s);
+    return scatter_batched_t<TL, 1, 1>(n, d, block_start, count, total_blocks, p, s);
+}
+
 // ------------------------------------------------------------------ launchers --
 template <int TL>
 hipError_t deposit_t(int n, int mode, const float* x, const float* y, const float* m, BinState bs, int max_chunks,
@@ -533,7 +672,7 @@ hipError_t push_deposit_k(int n, const float2* force, ParticleArrays p, BinState
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_push_deposit<TL, MODE, WRAP, COUNT, PIPE>), dim3(max_chunks), dim3(BLK), lds, s, force,
-                       p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, bs.chunk, scale, r, migrants,
+                       p.x, p.y, p.vx, p.vy, p.m, n, dt, bs.start, bs.chunk_start, scale, r, migrants,
                        bs.count);
     return hipGetLastError();
 }
@@ -666,6 +805,27 @@ hipError_t push(int n, int mode, const float2* force, float* x, float* y, float*
     if (mode == 0) return push_t<0, 1>(force, x, y, vx, vy, np, n, dt, s);
     if (wrap_mode == 0) return push_t<1, 0>(force, x, y, vx, vy, np, n, dt, s);
     return push_t<1, 1>(force, x, y, vx, vy, np, n, dt, s);
+}
+
+hipError_t push_deposit_batched(int n, int tl, int mode, int wrap_mode, bool count_bins, bool pipelined,
+                                const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
+                                hipStream_t s) {
+    if (count <= 0 || total_blocks <= 0) return hipSuccess;
+    if (tl == 5)
+        return push_deposit_batched_mode<5>(n, mode, wrap_mode, count_bins, pipelined, d, block_start, count,
+                                            total_blocks, p, s);
+    return push_deposit_batched_mode<6>(n, mode, wrap_mode, count_bins, pipelined, d, block_start, count,
+                                        total_blocks, p, s);
+}
+
+hipError_t push_deposit_scatter_batched(int n, int tl, int mode, int wrap_mode, const TiledDesc* d,
+                                        const int* block_start, int count, int total_blocks, int p, hipStream_t s) {
+    if (count <= 0 || total_blocks <= 0) return hipSuccess;
+    hipError_t e = tl == 5 ? scatter_batched_mode<5>(n, mode, wrap_mode, d, block_start, count, total_blocks, p, s)
+                           : scatter_batched_mode<6>(n, mode, wrap_mode, d, block_start, count, total_blocks, p, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_copy_parked_batched, dim3(64, count), dim3(kBlock), 0, s, d);
+    return hipGetLastError();
 }
 
 }  // namespace pmk

@@ -47,6 +47,11 @@ double now_s() { return std::chrono::duration<double>(Clock::now().time_since_ep
 // separate admission and read-back threads: 2 lanes 4.79-4.83 s, 3: 4.51,
 // 4: 4.45-4.46, 5: 4.74-4.76, 6: 4.56-4.57.
 constexpr int kLanes = 4;
+// Streams of a batched round: the members of each lattice size form a chain
+// of launches (FFT passes, then the particle kernels, step after step) that
+// does not depend on the other sizes' chains, so each runs on a stream of its
+// own (forked from the batch stream and joined back, inside the round graph).
+constexpr int kChains = 4;
 // Limit of one dependency wait of the persistent round kernel: a safety net,
 // far above any real wait (one pass of one member, microseconds).
 constexpr double kRoundWaitSeconds = 2.0;
@@ -77,7 +82,28 @@ class PinnedArena {
         if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return false;
         base_ = static_cast<char*>(p);
         bytes_ = bytes;
+        prefault();
         return true;
+    }
+    // The pages of the block are mapped on the first CPU write: a submission
+    // that first-touches its block copied 0.2 GB in 70 ms against 8 ms into a
+    // block touched before (2.9 against 25 GB/s), which made the 13.7 GB of
+    // the 512-request ensemble a ~4.7 s critical path of the submitting
+    // thread.  So every page is touched once here, by several threads.
+    void prefault() {
+        constexpr size_t kPage = 4096;
+        const unsigned hw = std::thread::hardware_concurrency();
+        const size_t nt = std::max(1u, std::min(16u, hw ? hw : 1u));
+        const size_t per = (bytes_ / nt + kPage - 1) / kPage * kPage;
+        std::vector<std::thread> ts;
+        for (size_t t = 0; t < nt; ++t) {
+            const size_t a = t * per, b = std::min(bytes_, a + per);
+            if (a >= b) break;
+            ts.emplace_back([this, a, b] {
+                for (size_t o = a; o < b; o += kPage) static_cast<volatile char*>(base_)[o] = 0;
+            });
+        }
+        for (auto& t : ts) t.join();
     }
     // a block of at least `need` bytes and its class, or nullptr
     char* take(size_t need, size_t* cls) {
@@ -135,7 +161,8 @@ struct Request {
     int32_t state = PM_REQ_QUEUED;
     std::string msg;
     pm_sim* sim = nullptr;
-    bool batched = false;
+    bool batched = false;   // runs in the batched launches ...
+    bool tiled = false;     // ... with the tile-sorted step kernels (above batch_max_particles)
     bool failed = false;    // set by the worker; the io thread reports it
     // captured rounds of a large simulation: graph[k] advances from gstate[k] to gstate[k ^ 1]
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -187,17 +214,32 @@ struct BatchPlan {
         size_t fdesc_off[2];
     };
     struct MGroup {
-        int mode;
+        int mode, n;
         std::vector<int> idx;
         size_t pdesc_off[2];
         size_t bstart_off;
         int total_blocks;
     };
+    // tiled members of one lattice size, tile size, deposit mode, wrap mode
+    // and loop variant; descriptors by round parity (their re-sort swaps the
+    // particle buffers and bin sets every round)
+    struct TGroup {
+        int n, tl, mode, wrap;
+        bool pipe;
+        std::vector<int> idx;
+        size_t tdesc_off[2];
+        size_t bstart_off;
+        int total_blocks, max_nbins;
+    };
     std::vector<NGroup> ngroups;
     std::vector<MGroup> mgroups;
+    std::vector<TGroup> tgroups;
     size_t fill_off[2] = {0, 0};
     size_t max_fill = 0;
-    hipGraphExec_t graph = nullptr;
+    // captured round by round parity since the plan was built (tiled members
+    // alternate between two states; the others are the same in both)
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int q = 0;
     // persistent round kernel over the members it runs (lattices up to 1024,
     // default plans); the others keep the per-pass launches
     std::vector<int> round_idx;
@@ -276,6 +318,8 @@ struct pm_server {
     double t_last_complete = 0;
     // streams (created once, default priority)
     hipStream_t batch_stream = nullptr, io_stream = nullptr, rb_stream = nullptr, lanes[kLanes] = {};
+    hipStream_t chains[kChains] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kChains] = {};
     std::thread worker, io, rb;
 };
 
@@ -349,6 +393,16 @@ void remember_spectra(pm_server* s, const pm_sim* h) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->bytes_in_use += (int64_t)b;
     s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_use);
+}
+
+// A simulation above batch_max_particles can join the batched launches when
+// its re-sort cycle is the round: the re-sort is fused into the step kernels
+// (interval >= 3, bin tables within LDS), the interval is the round length,
+// and it keeps the request's interval choice free.
+bool tiled_batchable(const pm_server* s, const Request* r, const pm_sim* h) {
+    const int R = std::max(2, s->cfg.round_steps & ~1);
+    return s->cfg.batch_tiled && r->cfg.resort_interval <= 0 && R >= 4 && h->resort_interval == R &&
+           h->fused_sort && pmk::scatter_fits(h->cfg.n_lattice, h->bins.tl);
 }
 
 // Admits the queue head if a slot and memory allow.  Returns false when it
@@ -431,16 +485,20 @@ bool admit_one(pm_server* s) {
     // never waits behind another's kernels; no host wait for it either (the
     // worker's streams wait on `admitted`)
     h->stream = h->own_stream;
-    // rounds are replayed from graphs with a fixed re-sort cadence
-    h->resort_permille = s->cfg.use_graphs ? 0 : 10;
+    // rounds are replayed from graphs, or batched, with a fixed re-sort cadence
+    const bool tiled_candidate = s->cfg.mode == PM_SERVE_BATCHED && s->cfg.batch_tiled &&
+                                 r->cfg.n_lattice <= s->cfg.batch_max_lattice && r->n > s->cfg.batch_max_particles &&
+                                 r->cfg.force_mode == PM_FORCE_SPECTRAL && r->cfg.resort_interval <= 0;
+    h->resort_permille = (s->cfg.use_graphs || tiled_candidate) ? 0 : 10;
     pm_config c = r->cfg;
     c.n_particles = h->cap;
     c.device = s->cfg.device;
     pm_status st = pmi::reconfigure(h, &c, cached_spectra(s, c));
-    // A large simulation's round is captured as a graph when the round is
-    // exactly one re-sort cycle: use the round length as the interval unless
-    // the request sets one (re-sorting never changes results).
-    if (st == PM_OK && s->cfg.use_graphs && r->cfg.resort_interval <= 0) {
+    // A large simulation's round is captured as a graph, or runs in the
+    // batched launches, when the round is exactly one re-sort cycle: use the
+    // round length as the interval unless the request sets one (re-sorting
+    // never changes results).
+    if (st == PM_OK && (s->cfg.use_graphs || tiled_candidate) && r->cfg.resort_interval <= 0) {
         h->resort_interval = s->cfg.round_steps;
         h->resort_auto = false;
     }
@@ -461,8 +519,11 @@ bool admit_one(pm_server* s) {
         s->slots_used -= 1;
         return true;
     }
-    r->batched = s->cfg.mode == PM_SERVE_BATCHED && r->cfg.n_lattice <= s->cfg.batch_max_lattice &&
-                 r->n <= s->cfg.batch_max_particles && r->cfg.force_mode == PM_FORCE_SPECTRAL;
+    const bool batchable = s->cfg.mode == PM_SERVE_BATCHED && r->cfg.n_lattice <= s->cfg.batch_max_lattice &&
+                           r->cfg.force_mode == PM_FORCE_SPECTRAL;
+    r->batched = batchable && r->n <= s->cfg.batch_max_particles;
+    r->tiled = batchable && !r->batched && tiled_batchable(s, r, h);
+    r->batched = r->batched || r->tiled;
     std::lock_guard<std::mutex> lk(s->mu);
     r->t_admit = now_s();
     r->state = PM_REQ_ACTIVE;
@@ -574,6 +635,37 @@ pmk::FieldDesc field_desc(const pm_sim* h, int rho_index) {
     return d;
 }
 
+// A tiled member's descriptor for round parity q (0: its current state; 1:
+// the state after one round, whose re-sort swaps the particle buffers and the
+// two bin sets).  The rounds have an even step count, so the current mass
+// lattice is the same at every round start; rho_next[p] is the deposit target
+// of the round's steps of parity p.
+pmk::TiledDesc tiled_desc(const pm_sim* h, int q) {
+    const pmi::HostState st = pmi::save_state(h);
+    const pmk::BinState& b = q == 0 ? st.bins : st.next;
+    const pmk::BinState& nb = q == 0 ? st.next : st.bins;
+    pmk::TiledDesc d{};
+    d.cur = q == 0 ? st.cur : st.alt;
+    d.alt = q == 0 ? st.alt : st.cur;
+    d.force = h->force;
+    for (int p = 0; p < 2; ++p)
+        d.rho_next[p] = reinterpret_cast<unsigned long long*>(h->rho_buf[h->rho_cur ^ p ^ 1]);
+    d.start = b.start;
+    d.chunk_start = b.chunk_start;
+    d.next_count = nb.count;
+    d.next_start = nb.start;
+    d.next_chunk_start = nb.chunk_start;
+    d.next_cursor = nb.cursor;
+    d.migrants = h->migrants;
+    d.n = h->cfg.n_lattice;
+    d.nbins = h->nbins;
+    d.chunk = b.chunk;
+    d.kmax = b.max_chunks_per_bin > 0 ? b.max_chunks_per_bin : (1 << 30);
+    d.dt = h->cfg.dt;
+    d.scale = std::ldexp(1.0f, h->shift);
+    return d;
+}
+
 // Device descriptor area, and two pinned host staging buffers for it used in
 // turn: a plan build reuses a staging buffer only after the copy made from
 // it two builds earlier has completed (its event).
@@ -607,12 +699,15 @@ bool ensure_scratch(pm_server* s, size_t bytes) {
 pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int steps,
                      std::vector<hipGraphExec_t>* retired) {
     BatchPlan& P = s->plan;
-    if (P.graph) { retired->push_back(P.graph); P.graph = nullptr; }
+    for (auto& g : P.graph)
+        if (g) { retired->push_back(g); g = nullptr; }
+    P.q = 0;
     P.members = members;
     P.signature.clear();
     for (Request* r : members) P.signature.push_back(r->id);
     P.ngroups.clear();
     P.mgroups.clear();
+    P.tgroups.clear();
     P.round_idx.clear();
     P.round_max_n = 0;
     for (int i = 0; i < (int)members.size(); ++i) {
@@ -628,9 +723,23 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
         });
         if (ng == P.ngroups.end()) P.ngroups.push_back({h->cfg.n_lattice, h->plan, {i}, {0, 0}});
         else ng->idx.push_back(i);
+        if (members[i]->tiled) {
+            const int n = h->cfg.n_lattice;
+            const bool pipe = h->np < (int64_t)pmk::kPipelineMaxDensity * n * n;
+            auto tg = std::find_if(P.tgroups.begin(), P.tgroups.end(), [&](const BatchPlan::TGroup& g) {
+                return g.n == n && g.tl == h->bins.tl && g.mode == h->cfg.mode && g.wrap == h->cfg.wrap_mode &&
+                       g.pipe == pipe;
+            });
+            if (tg == P.tgroups.end())
+                P.tgroups.push_back({n, h->bins.tl, h->cfg.mode, h->cfg.wrap_mode, pipe, {i}, {0, 0}, 0, 0, 0});
+            else tg->idx.push_back(i);
+            continue;
+        }
         auto mg = std::find_if(P.mgroups.begin(), P.mgroups.end(),
-                               [&](const BatchPlan::MGroup& g) { return g.mode == h->cfg.mode; });
-        if (mg == P.mgroups.end()) P.mgroups.push_back({h->cfg.mode, {i}, {0, 0}, 0, 0});
+                               [&](const BatchPlan::MGroup& g) {
+                                   return g.mode == h->cfg.mode && g.n == h->cfg.n_lattice;
+                               });
+        if (mg == P.mgroups.end()) P.mgroups.push_back({h->cfg.mode, h->cfg.n_lattice, {i}, {0, 0}, 0, 0});
         else mg->idx.push_back(i);
     }
     // layout
@@ -640,6 +749,10 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
         for (int p = 0; p < 2; ++p) g.fdesc_off[p] = take(sizeof(pmk::FieldDesc) * g.idx.size());
     for (auto& g : P.mgroups) {
         for (int p = 0; p < 2; ++p) g.pdesc_off[p] = take(sizeof(pmk::PartDesc) * g.idx.size());
+        g.bstart_off = take(sizeof(int) * (g.idx.size() + 1));
+    }
+    for (auto& g : P.tgroups) {
+        for (int q = 0; q < 2; ++q) g.tdesc_off[q] = take(sizeof(pmk::TiledDesc) * g.idx.size());
         g.bstart_off = take(sizeof(int) * (g.idx.size() + 1));
     }
     for (int p = 0; p < 2; ++p) P.fill_off[p] = take(sizeof(pmk::FillDesc) * members.size());
@@ -706,6 +819,21 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
         bs[g.idx.size()] = total;
         g.total_blocks = total;
     }
+    for (auto& g : P.tgroups) {
+        int* bs = reinterpret_cast<int*>(hb + g.bstart_off);
+        int total = 0;
+        g.max_nbins = 0;
+        for (size_t k = 0; k < g.idx.size(); ++k) {
+            const pm_sim* h = members[g.idx[k]]->sim;
+            bs[k] = total;
+            total += h->max_chunks;
+            g.max_nbins = std::max(g.max_nbins, h->nbins);
+            for (int q = 0; q < 2; ++q)   // q = 1: after one round (buffers and bin sets swapped)
+                reinterpret_cast<pmk::TiledDesc*>(hb + g.tdesc_off[q])[k] = tiled_desc(h, q);
+        }
+        bs[g.idx.size()] = total;
+        g.total_blocks = total;
+    }
     P.max_fill = 0;
     for (int p = 0; p < 2; ++p) {
         auto* d = reinterpret_cast<pmk::FillDesc*>(hb + P.fill_off[p]);
@@ -764,7 +892,7 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
     return PM_OK;
 }
 
-pm_status launch_batched_steps(pm_server* s, int steps) {
+pm_status launch_batched_steps(pm_server* s, int steps, int q) {
     BatchPlan& P = s->plan;
     char* db = static_cast<char*>(s->dscratch);
     if (!P.round_idx.empty() &&
@@ -774,24 +902,69 @@ pm_status launch_batched_steps(pm_server* s, int steps) {
                               P.round_max_n, P.round_lds, std::min(P.round_blocks, P.round_total), P.round_wait_ticks,
                               s->batch_stream) != hipSuccess)
         return PM_EDEVICE;
-    for (int st = 0; st < steps; ++st) {
-        const int p = st & 1;
-        for (auto& g : P.ngroups) {
-            const pm_sim* h0 = P.members[g.idx[0]]->sim;
-            const auto* d = reinterpret_cast<const pmk::FieldDesc*>(db + g.fdesc_off[p]);
-            const int cnt = (int)g.idx.size();
-            if (pmk::row_r2c(g.n, d, cnt, h0->tw, g.plan, s->batch_stream) != hipSuccess ||
-                pmk::col_conv(g.n, d, cnt, h0->tw, g.plan, true, s->batch_stream) != hipSuccess ||
-                pmk::row_c2r(g.n, d, cnt, h0->tw, g.plan, true, s->batch_stream) != hipSuccess ||
-                (!pmk::c2r_does_edges(g.plan) && pmk::edge_force(g.n, d, cnt, s->batch_stream) != hipSuccess))
-                return PM_EDEVICE;
+    // lattice sizes of the round, one chain each
+    std::vector<int> sizes;
+    for (auto& g : P.ngroups)
+        if (std::find(sizes.begin(), sizes.end(), g.n) == sizes.end()) sizes.push_back(g.n);
+    if (hipEventRecord(s->fork_ev, s->batch_stream) != hipSuccess) return PM_EDEVICE;
+    bool used[kChains] = {};
+    for (size_t c = 0; c < sizes.size(); ++c) {
+        const int n = sizes[c];
+        const int ci = (int)(c % kChains);
+        hipStream_t cs = s->chains[ci];
+        if (!used[ci] && hipStreamWaitEvent(cs, s->fork_ev, 0) != hipSuccess) return PM_EDEVICE;
+        used[ci] = true;
+        for (int st = 0; st < steps; ++st) {
+            const int p = st & 1;
+            for (auto& g : P.ngroups) {
+                if (g.n != n) continue;
+                const pm_sim* h0 = P.members[g.idx[0]]->sim;
+                const auto* d = reinterpret_cast<const pmk::FieldDesc*>(db + g.fdesc_off[p]);
+                const int cnt = (int)g.idx.size();
+                if (pmk::row_r2c(g.n, d, cnt, h0->tw, g.plan, cs) != hipSuccess ||
+                    pmk::col_conv(g.n, d, cnt, h0->tw, g.plan, true, cs) != hipSuccess ||
+                    pmk::row_c2r(g.n, d, cnt, h0->tw, g.plan, true, cs) != hipSuccess ||
+                    (!pmk::c2r_does_edges(g.plan) && pmk::edge_force(g.n, d, cnt, cs) != hipSuccess))
+                    return PM_EDEVICE;
+            }
+            for (auto& g : P.mgroups) {
+                if (g.n != n) continue;
+                if (pmk::push_deposit_direct(g.mode, reinterpret_cast<const pmk::PartDesc*>(db + g.pdesc_off[p]),
+                                             reinterpret_cast<const int*>(db + g.bstart_off), (int)g.idx.size(),
+                                             g.total_blocks, cs) != hipSuccess)
+                    return PM_EDEVICE;
+            }
+            // tiled members: one re-sort cycle per round, as pm_step runs it
+            // with the round length as interval (the counting step at
+            // steps - 2, the scan, the re-sorting step last)
+            for (auto& g : P.tgroups) {
+                if (g.n != n) continue;
+                const auto* d = reinterpret_cast<const pmk::TiledDesc*>(db + g.tdesc_off[q]);
+                const int* bs = reinterpret_cast<const int*>(db + g.bstart_off);
+                const int cnt = (int)g.idx.size();
+                hipError_t e;
+                if (st < steps - 2) {
+                    e = pmk::push_deposit_batched(g.n, g.tl, g.mode, g.wrap, false, g.pipe, d, bs, cnt,
+                                                  g.total_blocks, p, cs);
+                } else if (st == steps - 2) {
+                    e = pmk::zero_next_counts_batched(d, cnt, g.max_nbins, cs);
+                    if (e == hipSuccess)
+                        e = pmk::push_deposit_batched(g.n, g.tl, g.mode, g.wrap, true, g.pipe, d, bs, cnt,
+                                                      g.total_blocks, p, cs);
+                    if (e == hipSuccess) e = pmk::scan_next_batched(d, cnt, cs);
+                } else {
+                    e = pmk::push_deposit_scatter_batched(g.n, g.tl, g.mode, g.wrap, d, bs, cnt, g.total_blocks, p,
+                                                          cs);
+                }
+                if (e != hipSuccess) return PM_EDEVICE;
+            }
         }
-        for (auto& g : P.mgroups) {
-            if (pmk::push_deposit_direct(g.mode, reinterpret_cast<const pmk::PartDesc*>(db + g.pdesc_off[p]),
-                                         reinterpret_cast<const int*>(db + g.bstart_off), (int)g.idx.size(),
-                                         g.total_blocks, s->batch_stream) != hipSuccess)
-                return PM_EDEVICE;
-        }
+    }
+    for (int ci = 0; ci < kChains; ++ci) {
+        if (!used[ci]) continue;
+        if (hipEventRecord(s->join_ev[ci], s->chains[ci]) != hipSuccess ||
+            hipStreamWaitEvent(s->batch_stream, s->join_ev[ci], 0) != hipSuccess)
+            return PM_EDEVICE;
     }
     return PM_OK;
 }
@@ -808,6 +981,24 @@ void advance_batched_state(pm_sim* h, int steps) {
     h->steps_since_sort = std::max(0, h->resort_interval - 1);
 }
 
+// The same for a tiled member: one whole re-sort cycle ran (buffers and bin
+// sets swapped, as pm_step's re-sorting step leaves them).
+void advance_tiled_state(pm_sim* h, int steps) {
+    std::swap(h->cur, h->alt);
+    std::swap(h->bins.count, h->nx_count);
+    std::swap(h->bins.start, h->nx_start);
+    std::swap(h->bins.chunk_start, h->nx_chunk_start);
+    std::swap(h->bins.cursor, h->nx_cursor);
+    h->scatter_pending = false;
+    h->resort_request = false;
+    h->rho_cur ^= (steps & 1);
+    h->rho_valid = true;
+    h->field_parity = h->rho_cur ^ 1;
+    h->steps_since_sort = 0;
+    h->sorts += 1;
+    h->pushes += steps;
+}
+
 pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, int steps,
                             std::vector<hipGraphExec_t>* retired) {
     std::vector<int64_t> sig;
@@ -816,28 +1007,34 @@ pm_status run_batched_round(pm_server* s, const std::vector<Request*>& members, 
     if (sig != s->plan.signature || s->plan.members.empty()) {
         if ((st = build_plan(s, members, steps, retired)) != PM_OK) return st;
     }
+    const int q = s->plan.q;
     if (s->cfg.use_graphs) {
-        if (!s->plan.graph) {
+        hipGraphExec_t& ge = s->plan.graph[q];
+        if (!ge) {
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(s->batch_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return PM_EDEVICE;
-            st = launch_batched_steps(s, steps);
+            st = launch_batched_steps(s, steps, q);
             const hipError_t e = hipStreamEndCapture(s->batch_stream, &g);
             if (st != PM_OK || e != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
                 return PM_EDEVICE;
             }
-            const hipError_t e2 = hipGraphInstantiate(&s->plan.graph, g, nullptr, nullptr, 0);
+            const hipError_t e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            if (e2 != hipSuccess) { s->plan.graph = nullptr; return PM_EDEVICE; }
+            if (e2 != hipSuccess) { ge = nullptr; return PM_EDEVICE; }
             { std::lock_guard<std::mutex> lk_(s->mu); s->graph_captures += 1; }
         }
-        if (hipGraphLaunch(s->plan.graph, s->batch_stream) != hipSuccess) return PM_EDEVICE;
+        if (hipGraphLaunch(ge, s->batch_stream) != hipSuccess) return PM_EDEVICE;
         { std::lock_guard<std::mutex> lk_(s->mu); s->graph_launches += 1; }
     } else {
-        if ((st = launch_batched_steps(s, steps)) != PM_OK) return st;
+        if ((st = launch_batched_steps(s, steps, q)) != PM_OK) return st;
     }
-    for (Request* r : members) advance_batched_state(r->sim, steps);
+    s->plan.q ^= 1;
+    for (Request* r : members) {
+        if (r->tiled) advance_tiled_state(r->sim, steps);
+        else advance_batched_state(r->sim, steps);
+    }
     { std::lock_guard<std::mutex> lk_(s->mu); s->batched_rounds += 1; }
     return PM_OK;
 }
@@ -1097,6 +1294,10 @@ void pm_server_config_default(pm_server_config* c) {
     c->batch_max_particles = 1 << 17;
     c->device = 0;
     c->pinned_staging_mib = 1024;
+    // the tile-sorted members in the batched launches measured slower on the
+    // 512-request ensemble (4.60-4.64 s against 4.28-4.39 s on the lanes): a
+    // batched round ends when its slowest lattice size's chain does
+    c->batch_tiled = 0;
 }
 
 int32_t pm_plan_admission(const int64_t* need_bytes, int32_t n_queued, int64_t free_bytes, int32_t max_new) {
@@ -1116,7 +1317,8 @@ static void free_server(pm_server* s) {
         }
     for (pm_sim* h : s->pool) pm_destroy(h);
     for (auto& kv : s->spectra) (void)hipFree(kv.second);
-    if (s->plan.graph) (void)hipGraphExecDestroy(s->plan.graph);
+    for (hipGraphExec_t g : s->plan.graph)
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto& f : s->inflight) {
         for (hipGraphExec_t g : f.retired) (void)hipGraphExecDestroy(g);
         for (hipEvent_t e : f.ev)
@@ -1134,6 +1336,11 @@ static void free_server(pm_server* s) {
         if (*q) (void)hipStreamDestroy(*q);
     for (hipStream_t& q : s->lanes)
         if (q) (void)hipStreamDestroy(q);
+    for (hipStream_t& q : s->chains)
+        if (q) (void)hipStreamDestroy(q);
+    if (s->fork_ev) (void)hipEventDestroy(s->fork_ev);
+    for (hipEvent_t e : s->join_ev)
+        if (e) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -1142,6 +1349,7 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     *out = nullptr;
     if (cfg->hbm_budget_bytes <= 0 || cfg->max_active <= 0 || cfg->round_steps <= 0 || cfg->pinned_staging_mib < 0 ||
         cfg->persistent != 0 ||   // the persistent round kernel is disabled (it did not complete on the device)
+        (cfg->batch_tiled != 0 && cfg->batch_tiled != 1) ||
         (cfg->mode != PM_SERVE_PLAIN && cfg->mode != PM_SERVE_BATCHED))
         return PM_EINVAL;
     if (pm_device_count() <= cfg->device || cfg->device < 0) return PM_ENODEV;
@@ -1156,7 +1364,10 @@ pm_status pm_server_create(const pm_server_config* cfg, pm_server_handle* out) {
     for (hipStream_t* q : {&s->batch_stream, &s->io_stream, &s->rb_stream})
         ok = ok && hipStreamCreateWithFlags(q, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& q : s->lanes) ok = ok && hipStreamCreateWithFlags(&q, hipStreamNonBlocking) == hipSuccess;
+    for (hipStream_t& q : s->chains) ok = ok && hipStreamCreateWithFlags(&q, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&s->batch_done, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (hipEvent_t& e : s->join_ev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipHostMalloc(reinterpret_cast<void**>(&s->round_err_host), sizeof(int) * kRoundErrSlots) == hipSuccess;
     if (!ok) {
         free_server(s);

@@ -332,7 +332,8 @@ class ServerConfig(ctypes.Structure):
         ("batch_max_particles", ctypes.c_int32),
         ("pinned_staging_mib", ctypes.c_int32),
         ("persistent", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("batch_tiled", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

@@ -13,12 +13,16 @@
  * Modes:
  *   PM_SERVE_PLAIN    one request at a time, each stepped on its own to the
  *                     end (the one-at-a-time loop baseline);
- *   PM_SERVE_BATCHED  many requests resident; small simulations (lattice up to
- *                     `batch_max_lattice`, particles up to
- *                     `batch_max_particles`) are advanced by batched launches
- *                     (one launch per pass for all of them), larger ones on
- *                     their own streams, concurrently, each round replayed
- *                     from captured HIP graphs when `use_graphs` is set.
+ *   PM_SERVE_BATCHED  many requests resident; simulations with lattices up to
+ *                     `batch_max_lattice` are advanced by batched launches (one
+ *                     launch per FFT pass and lattice size, one per particle
+ *                     kernel and group, for all of them): those with at most
+ *                     `batch_max_particles` particles by the direct kernel,
+ *                     the larger ones (`batch_tiled`) by the tile-sorted step
+ *                     kernels with their re-sort cycle aligned to the round;
+ *                     the others on their own streams, concurrently; rounds
+ *                     replayed from captured HIP graphs when `use_graphs` is
+ *                     set.
  *
  * Results do not depend on the mode or on which other requests share a
  * round: every simulation's arithmetic is the same as alone.
@@ -57,7 +61,12 @@ typedef struct pm_server_config {
                                   * Disabled in this release (the kernel did not complete its first
                                   * rounds on the device; cause not found): must be 0, any other
                                   * value is PM_EINVAL. */
-    int32_t reserved[5];
+    int32_t batch_tiled;         /* BATCHED (default 0): simulations above batch_max_particles with
+                                  * lattices up to batch_max_lattice also run in the batched launches,
+                                  * with the tile-sorted step kernels over descriptor tables (their
+                                  * re-sort interval is the round length; the request must not set its
+                                  * own); 0 = on their own streams */
+    int32_t reserved[4];
 } pm_server_config;
 
 typedef struct pm_server_stats {

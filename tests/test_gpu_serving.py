@@ -52,22 +52,28 @@ def mix():
     return reqs, want
 
 
-@pytest.mark.parametrize("mode,graphs,batch_max,pinned_mib,persistent", [
-    (pm.SERVE_BATCHED, 1, 512, 1024, 0),
-    (pm.SERVE_BATCHED, 0, 512, 1024, 0),
-    (pm.SERVE_BATCHED, 1, 0, 1024, 0),       # no batched launches: every request on its own stream
-    (pm.SERVE_PLAIN, 0, 512, 1024, 0),
-    (pm.SERVE_BATCHED, 1, 512, 0, 0),        # request arrays in pageable memory
-    (pm.SERVE_BATCHED, 1, 512, 2, 0),        # 2 MiB of pinned staging: some requests pinned, the rest pageable
-    pytest.param(pm.SERVE_BATCHED, 0, 512, 1024, 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
-    pytest.param(pm.SERVE_BATCHED, 1, 512, 1024, 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+@pytest.mark.parametrize("mode,graphs,batch_max,pinned_mib,persistent,max_particles,tiled", [
+    (pm.SERVE_BATCHED, 1, 512, 1024, 0, 131072, 0),
+    (pm.SERVE_BATCHED, 0, 512, 1024, 0, 131072, 0),
+    (pm.SERVE_BATCHED, 1, 0, 1024, 0, 131072, 0),       # no batched launches: every request on its own stream
+    (pm.SERVE_PLAIN, 0, 512, 1024, 0, 131072, 0),
+    (pm.SERVE_BATCHED, 1, 512, 0, 0, 131072, 0),        # request arrays in pageable memory
+    (pm.SERVE_BATCHED, 1, 512, 2, 0, 131072, 0),        # 2 MiB of pinned staging: some requests pinned, the rest pageable
+    # tile-sorted step kernels in the batched launches (every request above
+    # 10000 particles; the stencil-force request stays on its own stream)
+    (pm.SERVE_BATCHED, 1, 1024, 1024, 0, 10000, 1),
+    (pm.SERVE_BATCHED, 0, 1024, 1024, 0, 10000, 1),
+    (pm.SERVE_BATCHED, 1, 1024, 1024, 0, 10000, 0),     # the same requests on their own streams
+    pytest.param(pm.SERVE_BATCHED, 0, 512, 1024, 1, 131072, 0, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+    pytest.param(pm.SERVE_BATCHED, 1, 512, 1024, 1, 131072, 0, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
 ])
-def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max, pinned_mib, persistent):
+def test_mix_matches_alone_bitwise(mix, mode, graphs, batch_max, pinned_mib, persistent, max_particles, tiled):
     reqs, want = mix
     print(f"serving mode={mode} graphs={graphs} batch_max_lattice={batch_max} pinned={pinned_mib} MiB "
-          f"persistent={persistent}", flush=True)
+          f"persistent={persistent} batch_max_particles={max_particles} batch_tiled={tiled}", flush=True)
     with pm.Server(mode=mode, use_graphs=graphs, batch_max_lattice=batch_max, pinned_staging_mib=pinned_mib,
-                   persistent=persistent, hbm_budget_bytes=8 << 30) as srv:
+                   persistent=persistent, batch_max_particles=max_particles, batch_tiled=tiled,
+                   hbm_budget_bytes=8 << 30) as srv:
         ids = [srv.submit(c, *a, steps=st) for (c, a), (_, _, _, st, _) in zip(reqs, MIX)]
         got = [srv.wait(i, timeout_s=120) for i in ids]
         stats = srv.stats()
@@ -162,17 +168,18 @@ def c5():
     return reqs, cfgs, arrays, steps, want
 
 
-@pytest.mark.parametrize("arrivals,persistent", [
-    ("saturated", 0), ("stream", 0),
-    pytest.param("saturated", 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
-    pytest.param("stream", 1, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+@pytest.mark.parametrize("arrivals,persistent,tiled", [
+    ("saturated", 0, 0), ("stream", 0, 0),
+    ("saturated", 0, 1),      # requests above batch_max_particles in the batched launches (tiled)
+    pytest.param("saturated", 1, 0, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
+    pytest.param("stream", 1, 0, marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
 ])
-def test_config5_ensemble_matches_alone_bitwise(c5, arrivals, persistent):
+def test_config5_ensemble_matches_alone_bitwise(c5, arrivals, persistent, tiled):
     import time
     reqs, cfgs, arrays, steps, want = c5
     print(f"config 5 subset: {len(reqs)} requests, {sum(r['n'] for r in reqs)} particles, {arrivals}, "
-          f"persistent={persistent}", flush=True)
-    with pm.Server(mode=pm.SERVE_BATCHED, use_graphs=1, persistent=persistent) as srv:
+          f"persistent={persistent} batch_tiled={tiled}", flush=True)
+    with pm.Server(mode=pm.SERVE_BATCHED, use_graphs=1, persistent=persistent, batch_tiled=tiled) as srv:
         ids = []
         for c, a in zip(cfgs, arrays):
             ids.append(srv.submit(c, *a, steps=steps))

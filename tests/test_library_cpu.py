@@ -80,7 +80,7 @@ def test_server_config_defaults():
     c = pm.default_server_config()
     assert c.mode == pm.SERVE_BATCHED and c.round_steps == 8 and c.use_graphs == 1
     assert c.hbm_budget_bytes == 200 << 30 and c.max_active > 1
-    assert c.pinned_staging_mib == 1024 and c.persistent == 0
+    assert c.pinned_staging_mib == 1024 and c.persistent == 0 and c.batch_tiled == 0
 
 
 def test_plan_admission_is_strict_fifo():
@@ -100,7 +100,7 @@ def test_plan_admission_is_strict_fifo():
 def test_server_create_validates_before_device():
     h = ctypes.c_void_p()
     for kw in ({"hbm_budget_bytes": 0}, {"max_active": 0}, {"round_steps": 0}, {"mode": 5},
-               {"persistent": 1}, {"persistent": -1}):
+               {"persistent": 1}, {"persistent": -1}, {"batch_tiled": 2}, {"batch_tiled": -1}):
         c = pm.default_server_config(**kw)
         assert pm.lib().pm_server_create(ctypes.byref(c), ctypes.byref(h)) == pm.PM_EINVAL
     assert pm.lib().pm_server_destroy(None) == pm.PM_EINVAL

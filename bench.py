@@ -100,6 +100,33 @@ def cpu_baseline(n, mode, dt, arrs, steps, threads, shift):
     return len(x) * steps / el, el, orc.num_threads()
 
 
+def device_copy_rate(nbytes=2 << 30, reps=10):
+    """Bytes moved per second (read + write) by the HIP runtime's own
+    device-to-device copy of nbytes, HIP events around reps copies: what a
+    plain streaming kernel of the same device reaches, for scale."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    assert hip.hipMalloc(ctypes.byref(a), sz(nbytes)) == 0 and hip.hipMalloc(ctypes.byref(b), sz(nbytes)) == 0
+    assert hip.hipMemset(a, 1, sz(nbytes)) == 0
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpyAsync(b, a, sz(nbytes), 3, vp(0)) == 0          # hipMemcpyDeviceToDevice
+    hip.hipEventRecord(e0, vp(0))
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, sz(nbytes), 3, vp(0))
+    hip.hipEventRecord(e1, vp(0))
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    for x in (e0, e1):
+        hip.hipEventDestroy(x)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2 * nbytes * reps / (ms.value / 1e3) / 1e9
+
+
 def copy_probe(nbytes):
     """One device-to-device hipMemcpy of nbytes (a copy kernel of known size)."""
     import ctypes
@@ -570,11 +597,15 @@ def main():
             am_bytes += algorithmic_bytes(k, n, np_) * cnt
             am_ms += ms
     frac_am = round(am_bytes / (am_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if am_ms > 0 else None
+    copy_gbs = round(device_copy_rate(), 1)
     roofline = {"bound": "hbm", "achieved": kd["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(kd["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, "
                                   "committed; not measured in this run)" if traffic is not None else None,
                 "kernel": dom, "frac_amortised": frac_am,
+                "device_copy_GB_s": copy_gbs,
+                "device_copy_note": "hipMemcpy device to device of 2 GiB (read + write bytes per second) timed in "
+                                    "the same run: the rate of the runtime's own streaming copy kernel",
                 "frac_amortised_kernels": [k for k in step_kernels if prof.get(k, (0, 0))[1]]}
 
     value = np_ * args.steps / el

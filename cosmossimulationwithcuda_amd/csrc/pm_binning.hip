@@ -113,10 +113,12 @@ __global__ __launch_bounds__(kBlock) void k_zero_next_counts(const TiledDesc* __
 
 // Counting-sort scatter.  A block of kScatBlock threads takes kScatGroups
 // aligned groups of 4 particles per thread (16-byte source reads), ranks each
-// particle within its bin in LDS, claims per-bin ranges with one global
-// atomic per (block, bin), and moves x, y, vx, vy, m, id to their sorted
-// slots in the second buffer set.  Lanes of one LDS-atomic instruction that
-// share a bin receive consecutive ranks, so their writes are contiguous.
+// particle within its bin in LDS (one atomic per wave for the lanes in the
+// wave's first bin: nearly all of them when re-sorting particles that were
+// sorted a few steps ago), claims per-bin ranges with one global atomic per
+// (block, bin), and moves x, y, vx, vy, m, id to their sorted slots in the
+// second buffer set.  Lanes of one wave that share a bin receive consecutive
+// ranks, so their writes are contiguous.
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 constexpr int kScatBlock = 1024;
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(kScatBlock) void k_bin_scatter(ParticleArrays src, 
             bq[g][k] = -1;
             if (4 * q + k < np) {
                 bq[g][k] = bin_of(xq[g][k], yq[g][k], n, tl);
-                rq[g][k] = atomicAdd(&lcnt[bq[g][k]], 1);
+                rq[g][k] = wave_count_add(lcnt, bq[g][k]);
             }
         }
     }

@@ -70,6 +70,25 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// cnt[b] += 1 for every active lane (b >= 0); returns the value before this
+// lane's increment.  Lanes in one bin get consecutive values, so a block's
+// values for a bin are a permutation of 0 .. count - 1.  The lanes that share
+// the wave's first active bin are served by one LDS atomic (in a dense tile
+// nearly every lane of a wave is in the same bin, where per-lane atomics to
+// one address serialise); the others add individually.
+__device__ __forceinline__ int wave_count_add(int* cnt, int b) {
+    const unsigned long long active = __ballot(1);
+    const int b0 = __shfl(b, __ffsll((long long)active) - 1, 64);
+    const unsigned long long same = __ballot(b == b0);
+    if (b != b0) return atomicAdd(&cnt[b], 1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)same) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[b0], __popcll(same));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(same & ((1ull << lane) - 1ull));
+}
+
 // Bin of a position: its tile when in bounds (1 <= x, y < N-1, the
 // reference's rule, cuh:257, 497), else the parked bin (N >> tl)^2.
 __device__ __forceinline__ int bin_of(float x, float y, int n, int tl) {

@@ -42,6 +42,7 @@ MIX = [  # (lattice, particles, mode, steps, extra config)
     (1024, 200000, pm.CIC, 53, {}),
     (1024, 150000, pm.NGP, 16, {}),
     (512, 30000, pm.CIC, 21, {"force_mode": pm.FORCE_STENCIL}),
+    (512, 40000, pm.NGP, 29, {"wrap_mode": pm.WRAP_REFERENCE}),
 ]
 
 

@@ -77,16 +77,25 @@ __device__ __forceinline__ void lds_barrier() {
 // nearly every lane of a wave is in the same bin, where per-lane atomics to
 // one address serialise); the others add individually.
 __device__ __forceinline__ int wave_count_add(int* cnt, int b) {
-    const unsigned long long active = __ballot(1);
-    const int b0 = __shfl(b, __ffsll((long long)active) - 1, 64);
+    const int b0 = __builtin_amdgcn_readfirstlane(b);        // bin of the first active lane
     const unsigned long long same = __ballot(b == b0);
     if (b != b0) return atomicAdd(&cnt[b], 1);
     const int lane = (int)__lane_id();
-    const int leader = __ffsll((long long)same) - 1;
     int base = 0;
-    if (lane == leader) base = atomicAdd(&cnt[b0], __popcll(same));
-    base = __shfl(base, leader, 64);
+    if (lane == __ffsll((long long)same) - 1) base = atomicAdd(&cnt[b0], __popcll(same));
+    base = __builtin_amdgcn_readfirstlane(base);              // the first active lane is the leader
     return base + __popcll(same & ((1ull << lane) - 1ull));
+}
+
+// The same count without the ranks: no value returns from the atomics.
+__device__ __forceinline__ void wave_count(int* cnt, int b) {
+    const int b0 = __builtin_amdgcn_readfirstlane(b);
+    const unsigned long long same = __ballot(b == b0);
+    if (b != b0) {
+        atomicAdd(&cnt[b], 1);
+        return;
+    }
+    if ((int)__lane_id() == __ffsll((long long)same) - 1) atomicAdd(&cnt[b0], __popcll(same));
 }
 
 // Bin of a position: its tile when in bounds (1 <= x, y < N-1, the

@@ -106,6 +106,13 @@ __device__ __forceinline__ void push_one(const float2* __restrict__ force, int n
 // true), else nothing (returns false) and the caller spills it to the lattice
 // with deposit_global.  The two paths are kept in separate loops so the
 // compiler cannot merge an LDS and a global atomic into one flat atomic.
+// Whether a deposit at (px, py) lands inside the W x W tile window at
+// (wx0, wy0): the test deposit_local makes before it deposits.
+template <int W>
+__device__ __forceinline__ bool in_window(int wx0, int wy0, float px, float py) {
+    return (unsigned)((int)px - wx0) < (unsigned)(W - 1) && (unsigned)((int)py - wy0) < (unsigned)(W - 1);
+}
+
 template <int MODE, int W>
 __device__ __forceinline__ bool deposit_local(unsigned long long* win, int wx0, int wy0, float px, float py,
                                               float pm, float scale) {

@@ -112,7 +112,12 @@ template <int TL> constexpr int fused_block() { return TL <= 5 ? 512 : 1024; }
 
 // Chunk c of one simulation; the single-simulation kernel runs chunk
 // blockIdx.x, the batched one (k_push_deposit_batched) maps its blocks over
-// the chunks of several simulations.
+// the chunks of several simulations.  A chunk of fewer than kSparseChunk
+// particles (a sparse tile of a small or spread-out simulation) gathers from
+// the force lattice and deposits with global atomics instead of staging the
+// two tile windows in LDS: the same arithmetic, bit for bit, without loading
+// and flushing ~20 KB of windows for a few dozen particles.
+constexpr int kSparseChunk = 256;
 template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
 __device__ __forceinline__ void push_deposit_block(
     const float2* __restrict__ force, float* __restrict__ x, float* __restrict__ y, float* __restrict__ vx,
@@ -128,13 +133,15 @@ __device__ __forceinline__ void push_deposit_block(
     int b;
     int64_t s0, s1;
     if (!chunk_of(chunk_start, start, nin, c, b, s0, s1)) return;
+    const bool sparse = s1 - s0 < kSparseChunk;
     const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
-    for (int e = threadIdx.x; e < W * W; e += BLK) {
-        win[e] = 0ull;
-        const int gx = wx0 + e % W, gy = wy0 + e / W;
-        fwin[e] = ((unsigned)gx < (unsigned)n && (unsigned)gy < (unsigned)n) ? force[(size_t)gy * n + gx]
-                                                                             : make_float2(0.f, 0.f);
-    }
+    if (!sparse)
+        for (int e = threadIdx.x; e < W * W; e += BLK) {
+            win[e] = 0ull;
+            const int gx = wx0 + e % W, gy = wy0 + e / W;
+            fwin[e] = ((unsigned)gx < (unsigned)n && (unsigned)gy < (unsigned)n) ? force[(size_t)gy * n + gx]
+                                                                                 : make_float2(0.f, 0.f);
+        }
     if (COUNT)
         for (int e = threadIdx.x; e <= nin; e += BLK) hist[e] = 0;
     if (threadIdx.x == 0) s_mig = 0;
@@ -216,7 +223,7 @@ __device__ __forceinline__ void push_deposit_block(
                 const int lx = xi - wx0, ly = yi - wy0;
                 const float fx = frac(px[u][k]), fy = frac(py[u][k]);
                 float2 a;
-                if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
+                if (!sparse && (unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
                     a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
                 else
                     a = gather<MODE>(force + (size_t)yi * n + xi, n, fx, fy);
@@ -254,20 +261,25 @@ __device__ __forceinline__ void push_deposit_block(
             for (int k = 0; k < 4; ++k) {
                 spill[u][k] = false;
                 if (!ok[u][k]) continue;
-                if (COUNT) (void)wave_count_add(hist, bin_of(px[u][k], py[u][k], n, TL));
+                if (COUNT) wave_count(hist, bin_of(px[u][k], py[u][k], n, TL));
                 spill[u][k] = in_bounds(px[u][k], py[u][k], fhi) &&
-                              !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale);
+                              (sparse || !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale));
             }
         }
+        // (a sparse chunk deposits everything globally; its migrants are the
+        // particles outside the window, as in a dense chunk)
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (spill[u][k]) { deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale); ++mig; }
+                if (spill[u][k]) {
+                    deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale);
+                    mig += !sparse || !in_window<W>(wx0, wy0, px[u][k], py[u][k]);
+                }
     }
     if (mig) atomicAdd(&s_mig, mig);
     lds_barrier();
-    flush_window<W, BLK>(win, wx0, wy0, rho, n);
+    if (!sparse) flush_window<W, BLK>(win, wx0, wy0, rho, n);
     if (COUNT)
         for (int e = threadIdx.x; e <= nin; e += BLK)
             if (hist[e]) atomicAdd(&count[e], hist[e]);
@@ -336,13 +348,15 @@ __device__ __forceinline__ void push_deposit_scatter_block(
     int b;
     int64_t s0, s1;
     if (!chunk_of(chunk_start, start, nin, c, b, s0, s1)) return;
+    const bool sparse = s1 - s0 < kSparseChunk;       // gather and deposit through global memory
     const int wx0 = (b % tpr) * E - kHalo, wy0 = (b / tpr) * E - kHalo;
-    for (int e = threadIdx.x; e < W * W; e += BLK) {
-        win[e] = 0ull;
-        const int gx = wx0 + e % W, gy = wy0 + e / W;
-        fwin[e] = ((unsigned)gx < (unsigned)n && (unsigned)gy < (unsigned)n) ? force[(size_t)gy * n + gx]
-                                                                             : make_float2(0.f, 0.f);
-    }
+    if (!sparse)
+        for (int e = threadIdx.x; e < W * W; e += BLK) {
+            win[e] = 0ull;
+            const int gx = wx0 + e % W, gy = wy0 + e / W;
+            fwin[e] = ((unsigned)gx < (unsigned)n && (unsigned)gy < (unsigned)n) ? force[(size_t)gy * n + gx]
+                                                                                 : make_float2(0.f, 0.f);
+        }
     for (int e = threadIdx.x; e <= nin; e += BLK) lcnt[e] = 0;
     if (threadIdx.x == 0) s_mig = 0;
     lds_barrier();
@@ -403,7 +417,7 @@ __device__ __forceinline__ void push_deposit_scatter_block(
                 const int lx = xi - wx0, ly = yi - wy0;
                 const float fx = frac(px[u][k]), fy = frac(py[u][k]);
                 float2 a;
-                if ((unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
+                if (!sparse && (unsigned)lx < (unsigned)(W - 1) && (unsigned)ly < (unsigned)(W - 1))
                     a = gather<MODE>(fwin + ly * W + lx, W, fx, fy);
                 else
                     a = gather<MODE>(force + (size_t)yi * n + xi, n, fx, fy);
@@ -416,14 +430,17 @@ __device__ __forceinline__ void push_deposit_scatter_block(
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 spill[u][k] = ok[u][k] && in_bounds(px[u][k], py[u][k], fhi) &&
-                              !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale);
+                              (sparse || !deposit_local<MODE, W>(win, wx0, wy0, px[u][k], py[u][k], pm[u][k], scale));
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (spill[u][k]) { deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale); ++mig; }
+                if (spill[u][k]) {
+                    deposit_global<MODE>(rho, n, px[u][k], py[u][k], pm[u][k], scale);
+                    mig += !sparse || !in_window<W>(wx0, wy0, px[u][k], py[u][k]);
+                }
         lds_barrier();                                   // ranks of this pass complete
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -454,7 +471,7 @@ __device__ __forceinline__ void push_deposit_scatter_block(
     }
     if (mig) atomicAdd(&s_mig, mig);
     lds_barrier();
-    flush_window<W, BLK>(win, wx0, wy0, rho, n);
+    if (!sparse) flush_window<W, BLK>(win, wx0, wy0, rho, n);
     if (threadIdx.x == 0 && s_mig) atomicAdd(migrants, (unsigned long long)s_mig);
 }
 

@@ -298,17 +298,23 @@ struct TiledDesc {
     int n, nbins, chunk, kmax;            // lattice, bins incl. the parked one, deposit chunk, chunks per bin cap
     float dt, scale;                      // time step, 2^shift
 };
-// block_start[i]: first block of simulation i (exclusive prefix of its
-// max_chunks), block_start[count] = total_blocks.  p: parity of the step
-// (rho_next[p] is the deposit target).  count_bins: the counting step
-// (histogram of the new bins into next_count, zero on entry).
-hipError_t push_deposit_batched(int n, int tl, int mode, int wrap_mode, bool count_bins, bool pipelined,
-                                const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
-                                hipStream_t s);
+// chunk_first[i]: chunks of simulations 0 .. i-1 in their current chunk maps,
+// chunk_first[count] their total (count + 1 entries, written on the device;
+// run again whenever the chunk maps change: after each re-sort).
+hipError_t member_chunks_batched(const TiledDesc* d, int count, int* chunk_first, hipStream_t s);
+// chunk_first: as member_chunks_batched left it for these descriptors;
+// total_blocks: an upper bound of the chunks (the sum of the members'
+// max_chunks), which bounds the grid.  p: parity of the step (rho_next[p] is
+// the deposit target).  count_bins: the counting step (histogram of the new
+// bins into next_count, zero on entry).  The loop is the one loading at the
+// top of each pass (the software-pipelined one needs more registers than two
+// blocks per CU leave once blocks stride over chunks).
+hipError_t push_deposit_batched(int n, int tl, int mode, int wrap_mode, bool count_bins, const TiledDesc* d,
+                                const int* chunk_first, int count, int total_blocks, int p, hipStream_t s);
 // The re-sorting step (cur -> alt in the order of the counted bins, with the
 // standing parked particles) of every simulation.
 hipError_t push_deposit_scatter_batched(int n, int tl, int mode, int wrap_mode, const TiledDesc* d,
-                                        const int* block_start, int count, int total_blocks, int p, hipStream_t s);
+                                        const int* chunk_first, int count, int total_blocks, int p, hipStream_t s);
 // next_count = 0 (nbins entries) of every simulation.
 hipError_t zero_next_counts_batched(const TiledDesc* d, int count, int max_nbins, hipStream_t s);
 // scan_next of every simulation (next bins from next_count; old starts = start).

@@ -296,30 +296,42 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit(
                                                     migrants, count, blockIdx.x);
 }
 
-// Simulation of block blockIdx.x in a batched launch: the last i with
-// block_start[i] <= blockIdx.x (block_start[count] = total blocks).
-__device__ __forceinline__ int batch_member(const int* __restrict__ block_start, int count) {
+// Most blocks of a batched step launch (the blocks stride over the members'
+// chunks): about four per resident block slot at two per CU.
+constexpr int kBatchedGrid = 2048;
+
+// Simulation of chunk b of a batched launch: the last i with
+// chunk_first[i] <= b (chunk_first: exclusive prefix of the members' chunk
+// counts, b below their total).
+__device__ __forceinline__ int batch_member(const int* __restrict__ chunk_first, int count, int b) {
     int lo = 0, hi = count;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (block_start[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+        if (chunk_first[mid] <= b) lo = mid; else hi = mid;
     }
     return lo;
 }
 
 // The step kernel over several simulations of one lattice size, deposit mode
-// and wrap mode (a batched serving round): the blocks of simulation i are
-// block_start[i] .. block_start[i + 1] - 1, one per possible chunk of it.
+// and wrap mode (a batched serving round).  The chunks of all members are
+// numbered consecutively (chunk_first, filled on the device from the members'
+// chunk maps by k_member_chunks) and the blocks stride over them: the grid
+// holds at most kBatchedGrid blocks whatever the members' worst-case chunk
+// counts, and no block is spent on chunks that do not exist.
 // p: the step's parity (which mass lattice is the deposit target).
-template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
+template <int TL, int MODE, int WRAP, bool COUNT>
 __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_batched(const TiledDesc* __restrict__ descs,
-                                                                           const int* __restrict__ block_start,
+                                                                           const int* __restrict__ chunk_first,
                                                                            int count, int p) {
-    const int i = batch_member(block_start, count);
-    const TiledDesc& d = descs[i];
-    push_deposit_block<TL, MODE, WRAP, COUNT, PIPE>(d.force, d.cur.x, d.cur.y, d.cur.vx, d.cur.vy, d.cur.m, d.n, d.dt,
-                                                    d.start, d.chunk_start, d.scale, d.rho_next[p], d.migrants,
-                                                    d.next_count, (int)blockIdx.x - block_start[i]);
+    const int total = chunk_first[count];
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        const int i = batch_member(chunk_first, count, b);
+        const TiledDesc& d = descs[i];
+        push_deposit_block<TL, MODE, WRAP, COUNT, false>(d.force, d.cur.x, d.cur.y, d.cur.vx, d.cur.vy, d.cur.m, d.n,
+                                                        d.dt, d.start, d.chunk_start, d.scale, d.rho_next[p],
+                                                        d.migrants, d.next_count, b - chunk_first[i]);
+        lds_barrier();                               // the windows are reused by the block's next chunk
+    }
 }
 
 // --------------------------------------------- fused step kernel with re-sort --
@@ -489,12 +501,40 @@ __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter(
 // block mapping as k_push_deposit_batched.
 template <int TL, int MODE, int WRAP>
 __global__ __launch_bounds__(fused_block<TL>()) void k_push_deposit_scatter_batched(
-    const TiledDesc* __restrict__ descs, const int* __restrict__ block_start, int count, int p) {
-    const int i = batch_member(block_start, count);
-    const TiledDesc& d = descs[i];
-    push_deposit_scatter_block<TL, MODE, WRAP>(d.force, d.cur, d.alt, d.n, d.dt, d.start, d.chunk_start,
-                                               d.next_start, d.next_cursor, d.scale, d.rho_next[p], d.migrants,
-                                               (int)blockIdx.x - block_start[i]);
+    const TiledDesc* __restrict__ descs, const int* __restrict__ chunk_first, int count, int p) {
+    const int total = chunk_first[count];
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        const int i = batch_member(chunk_first, count, b);
+        const TiledDesc& d = descs[i];
+        push_deposit_scatter_block<TL, MODE, WRAP>(d.force, d.cur, d.alt, d.n, d.dt, d.start, d.chunk_start,
+                                                   d.next_start, d.next_cursor, d.scale, d.rho_next[p], d.migrants,
+                                                   b - chunk_first[i]);
+        lds_barrier();                               // LDS tables and windows are reused by the next chunk
+    }
+}
+
+// chunk_first[i] = chunks of members 0 .. i-1 in their current chunk maps
+// (chunk_start[nin] of each), chunk_first[count] = the total: one block.
+__global__ __launch_bounds__(kBlock) void k_member_chunks(const TiledDesc* __restrict__ descs, int count,
+                                                          int* __restrict__ chunk_first) {
+    __shared__ int sc[kBlock];
+    int off = 0;
+    for (int base = 0; base < count; base += kBlock) {
+        const int k = base + threadIdx.x;
+        const int v = k < count ? descs[k].chunk_start[descs[k].nbins - 1] : 0;
+        sc[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < kBlock; o <<= 1) {       // inclusive scan
+            const int t = threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (k < count) chunk_first[k] = off + sc[threadIdx.x] - v;
+        off += sc[kBlock - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chunk_first[count] = off;
 }
 
 // Particles standing in the parked bin (never visited by the step kernels)
@@ -586,64 +626,56 @@ __global__ __launch_bounds__(kBlock) void k_copy_parked_batched(const TiledDesc*
 
 // Batched launchers: dynamic LDS as the single-simulation launchers (the bin
 // histogram of the counting step; the scatter's two bin tables).
-template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
-hipError_t push_deposit_batched_k(int n, const TiledDesc* d, const int* block_start, int count, int total_blocks,
+template <int TL, int MODE, int WRAP, bool COUNT>
+hipError_t push_deposit_batched_k(int n, const TiledDesc* d, const int* chunk_first, int count, int total_blocks,
                                   int p, hipStream_t s) {
     constexpr int W = TileGeom<TL>::kWin, BLK = fused_block<TL>();
     const size_t lds = COUNT ? sizeof(int) * ((n >> TL) * (n >> TL) + 1) : 0;
     if (COUNT && lds + 16 * W * W > 65536) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&k_push_deposit_batched<TL, MODE, WRAP, COUNT, PIPE>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_batched<TL, MODE, WRAP, COUNT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_push_deposit_batched<TL, MODE, WRAP, COUNT, PIPE>), dim3(total_blocks), dim3(BLK), lds, s,
-                       d, block_start, count, p);
+    hipLaunchKernelGGL((k_push_deposit_batched<TL, MODE, WRAP, COUNT>), dim3(std::min(total_blocks, kBatchedGrid)),
+                       dim3(BLK), lds, s, d, chunk_first, count, p);
     return hipGetLastError();
 }
 
-template <int TL, int MODE, int WRAP>
-hipError_t push_deposit_batched_t(int n, bool count_bins, bool pipelined, const TiledDesc* d, const int* block_start,
-                                  int count, int total_blocks, int p, hipStream_t s) {
-    if (count_bins)
-        return pipelined ? push_deposit_batched_k<TL, MODE, WRAP, true, true>(n, d, block_start, count, total_blocks, p, s)
-                         : push_deposit_batched_k<TL, MODE, WRAP, true, false>(n, d, block_start, count, total_blocks, p, s);
-    return pipelined ? push_deposit_batched_k<TL, MODE, WRAP, false, true>(n, d, block_start, count, total_blocks, p, s)
-                     : push_deposit_batched_k<TL, MODE, WRAP, false, false>(n, d, block_start, count, total_blocks, p, s);
-}
-
 template <int TL>
-hipError_t push_deposit_batched_mode(int n, int mode, int wrap_mode, bool count_bins, bool pipelined,
-                                     const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
-                                     hipStream_t s) {
+hipError_t push_deposit_batched_mode(int n, int mode, int wrap_mode, bool count_bins, const TiledDesc* d,
+                                     const int* chunk_first, int count, int total_blocks, int p, hipStream_t s) {
     if (mode == 0 && wrap_mode == 0)
-        return push_deposit_batched_t<TL, 0, 0>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+        return count_bins ? push_deposit_batched_k<TL, 0, 0, true>(n, d, chunk_first, count, total_blocks, p, s)
+                          : push_deposit_batched_k<TL, 0, 0, false>(n, d, chunk_first, count, total_blocks, p, s);
     if (mode == 0)
-        return push_deposit_batched_t<TL, 0, 1>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+        return count_bins ? push_deposit_batched_k<TL, 0, 1, true>(n, d, chunk_first, count, total_blocks, p, s)
+                          : push_deposit_batched_k<TL, 0, 1, false>(n, d, chunk_first, count, total_blocks, p, s);
     if (wrap_mode == 0)
-        return push_deposit_batched_t<TL, 1, 0>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
-    return push_deposit_batched_t<TL, 1, 1>(n, count_bins, pipelined, d, block_start, count, total_blocks, p, s);
+        return count_bins ? push_deposit_batched_k<TL, 1, 0, true>(n, d, chunk_first, count, total_blocks, p, s)
+                          : push_deposit_batched_k<TL, 1, 0, false>(n, d, chunk_first, count, total_blocks, p, s);
+    return count_bins ? push_deposit_batched_k<TL, 1, 1, true>(n, d, chunk_first, count, total_blocks, p, s)
+                      : push_deposit_batched_k<TL, 1, 1, false>(n, d, chunk_first, count, total_blocks, p, s);
 }
 
 template <int TL, int MODE, int WRAP>
-hipError_t scatter_batched_t(int n, const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
+hipError_t scatter_batched_t(int n, const TiledDesc* d, const int* chunk_first, int count, int total_blocks, int p,
                              hipStream_t s) {
     const size_t lds = 2 * sizeof(int) * (size_t)(((n >> TL) * (n >> TL)) + 1);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_scatter_batched<TL, MODE, WRAP>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_push_deposit_scatter_batched<TL, MODE, WRAP>), dim3(total_blocks), dim3(fused_block<TL>()),
-                       lds, s, d, block_start, count, p);
+    hipLaunchKernelGGL((k_push_deposit_scatter_batched<TL, MODE, WRAP>), dim3(std::min(total_blocks, kBatchedGrid)),
+                       dim3(fused_block<TL>()), lds, s, d, chunk_first, count, p);
     return hipGetLastError();
 }
 
 template <int TL>
-hipError_t scatter_batched_mode(int n, int mode, int wrap, const TiledDesc* d, const int* block_start, int count,
+hipError_t scatter_batched_mode(int n, int mode, int wrap, const TiledDesc* d, const int* chunk_first, int count,
                                 int total_blocks, int p, hipStream_t s) {
-    if (mode == 0 && wrap == 0) return scatter_batched_t<TL, 0, 0>(n, d, block_start, count, total_blocks, p, s);
-    if (mode == 0) return scatter_batched_t<TL, 0, 1>(n, d, block_start, count, total_blocks, p, s);
-    if (wrap == 0) return scatter_batched_t<TL, 1, 0>(n, d, block_start, count, total_blocks, p, s);
-    return scatter_batched_t<TL, 1, 1>(n, d, block_start, count, total_blocks, p, s);
+    if (mode == 0 && wrap == 0) return scatter_batched_t<TL, 0, 0>(n, d, chunk_first, count, total_blocks, p, s);
+    if (mode == 0) return scatter_batched_t<TL, 0, 1>(n, d, chunk_first, count, total_blocks, p, s);
+    if (wrap == 0) return scatter_batched_t<TL, 1, 0>(n, d, chunk_first, count, total_blocks, p, s);
+    return scatter_batched_t<TL, 1, 1>(n, d, chunk_first, count, total_blocks, p, s);
 }
 
 // ------------------------------------------------------------------ launchers --
@@ -805,22 +837,25 @@ hipError_t push(int n, int mode, const float2* force, float* x, float* y, float*
     return push_t<1, 1>(force, x, y, vx, vy, np, n, dt, s);
 }
 
-hipError_t push_deposit_batched(int n, int tl, int mode, int wrap_mode, bool count_bins, bool pipelined,
-                                const TiledDesc* d, const int* block_start, int count, int total_blocks, int p,
-                                hipStream_t s) {
+hipError_t push_deposit_batched(int n, int tl, int mode, int wrap_mode, bool count_bins, const TiledDesc* d,
+                                const int* chunk_first, int count, int total_blocks, int p, hipStream_t s) {
     if (count <= 0 || total_blocks <= 0) return hipSuccess;
     if (tl == 5)
-        return push_deposit_batched_mode<5>(n, mode, wrap_mode, count_bins, pipelined, d, block_start, count,
-                                            total_blocks, p, s);
-    return push_deposit_batched_mode<6>(n, mode, wrap_mode, count_bins, pipelined, d, block_start, count,
-                                        total_blocks, p, s);
+        return push_deposit_batched_mode<5>(n, mode, wrap_mode, count_bins, d, chunk_first, count, total_blocks, p, s);
+    return push_deposit_batched_mode<6>(n, mode, wrap_mode, count_bins, d, chunk_first, count, total_blocks, p, s);
+}
+
+hipError_t member_chunks_batched(const TiledDesc* d, int count, int* chunk_first, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_member_chunks, dim3(1), dim3(kBlock), 0, s, d, count, chunk_first);
+    return hipGetLastError();
 }
 
 hipError_t push_deposit_scatter_batched(int n, int tl, int mode, int wrap_mode, const TiledDesc* d,
-                                        const int* block_start, int count, int total_blocks, int p, hipStream_t s) {
+                                        const int* chunk_first, int count, int total_blocks, int p, hipStream_t s) {
     if (count <= 0 || total_blocks <= 0) return hipSuccess;
-    hipError_t e = tl == 5 ? scatter_batched_mode<5>(n, mode, wrap_mode, d, block_start, count, total_blocks, p, s)
-                           : scatter_batched_mode<6>(n, mode, wrap_mode, d, block_start, count, total_blocks, p, s);
+    hipError_t e = tl == 5 ? scatter_batched_mode<5>(n, mode, wrap_mode, d, chunk_first, count, total_blocks, p, s)
+                           : scatter_batched_mode<6>(n, mode, wrap_mode, d, chunk_first, count, total_blocks, p, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_copy_parked_batched, dim3(64, count), dim3(kBlock), 0, s, d);
     return hipGetLastError();

@@ -225,10 +225,9 @@ struct BatchPlan {
     // particle buffers and bin sets every round)
     struct TGroup {
         int n, tl, mode, wrap;
-        bool pipe;
         std::vector<int> idx;
         size_t tdesc_off[2];
-        size_t bstart_off;
+        size_t cfirst_off;                // members' chunk prefix, written on the device each round
         int total_blocks, max_nbins;
     };
     std::vector<NGroup> ngroups;
@@ -725,13 +724,11 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
         else ng->idx.push_back(i);
         if (members[i]->tiled) {
             const int n = h->cfg.n_lattice;
-            const bool pipe = h->np < (int64_t)pmk::kPipelineMaxDensity * n * n;
             auto tg = std::find_if(P.tgroups.begin(), P.tgroups.end(), [&](const BatchPlan::TGroup& g) {
-                return g.n == n && g.tl == h->bins.tl && g.mode == h->cfg.mode && g.wrap == h->cfg.wrap_mode &&
-                       g.pipe == pipe;
+                return g.n == n && g.tl == h->bins.tl && g.mode == h->cfg.mode && g.wrap == h->cfg.wrap_mode;
             });
             if (tg == P.tgroups.end())
-                P.tgroups.push_back({n, h->bins.tl, h->cfg.mode, h->cfg.wrap_mode, pipe, {i}, {0, 0}, 0, 0, 0});
+                P.tgroups.push_back({n, h->bins.tl, h->cfg.mode, h->cfg.wrap_mode, {i}, {0, 0}, 0, 0, 0});
             else tg->idx.push_back(i);
             continue;
         }
@@ -753,7 +750,7 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
     }
     for (auto& g : P.tgroups) {
         for (int q = 0; q < 2; ++q) g.tdesc_off[q] = take(sizeof(pmk::TiledDesc) * g.idx.size());
-        g.bstart_off = take(sizeof(int) * (g.idx.size() + 1));
+        g.cfirst_off = take(sizeof(int) * (g.idx.size() + 1));
     }
     for (int p = 0; p < 2; ++p) P.fill_off[p] = take(sizeof(pmk::FillDesc) * members.size());
     const int nr = (int)P.round_idx.size();
@@ -820,18 +817,15 @@ pm_status build_plan(pm_server* s, const std::vector<Request*>& members, int ste
         g.total_blocks = total;
     }
     for (auto& g : P.tgroups) {
-        int* bs = reinterpret_cast<int*>(hb + g.bstart_off);
         int total = 0;
         g.max_nbins = 0;
         for (size_t k = 0; k < g.idx.size(); ++k) {
             const pm_sim* h = members[g.idx[k]]->sim;
-            bs[k] = total;
             total += h->max_chunks;
             g.max_nbins = std::max(g.max_nbins, h->nbins);
             for (int q = 0; q < 2; ++q)   // q = 1: after one round (buffers and bin sets swapped)
                 reinterpret_cast<pmk::TiledDesc*>(hb + g.tdesc_off[q])[k] = tiled_desc(h, q);
         }
-        bs[g.idx.size()] = total;
         g.total_blocks = total;
     }
     P.max_fill = 0;
@@ -940,16 +934,18 @@ pm_status launch_batched_steps(pm_server* s, int steps, int q) {
             for (auto& g : P.tgroups) {
                 if (g.n != n) continue;
                 const auto* d = reinterpret_cast<const pmk::TiledDesc*>(db + g.tdesc_off[q]);
-                const int* bs = reinterpret_cast<const int*>(db + g.bstart_off);
+                int* bs = reinterpret_cast<int*>(db + g.cfirst_off);
                 const int cnt = (int)g.idx.size();
-                hipError_t e;
+                hipError_t e = hipSuccess;
+                if (st == 0) e = pmk::member_chunks_batched(d, cnt, bs, cs);   // the chunk maps of this round
+                if (e != hipSuccess) return PM_EDEVICE;
                 if (st < steps - 2) {
-                    e = pmk::push_deposit_batched(g.n, g.tl, g.mode, g.wrap, false, g.pipe, d, bs, cnt,
+                    e = pmk::push_deposit_batched(g.n, g.tl, g.mode, g.wrap, false, d, bs, cnt,
                                                   g.total_blocks, p, cs);
                 } else if (st == steps - 2) {
                     e = pmk::zero_next_counts_batched(d, cnt, g.max_nbins, cs);
                     if (e == hipSuccess)
-                        e = pmk::push_deposit_batched(g.n, g.tl, g.mode, g.wrap, true, g.pipe, d, bs, cnt,
+                        e = pmk::push_deposit_batched(g.n, g.tl, g.mode, g.wrap, true, d, bs, cnt,
                                                       g.total_blocks, p, cs);
                     if (e == hipSuccess) e = pmk::scan_next_batched(d, cnt, cs);
                 } else {
@@ -1294,9 +1290,8 @@ void pm_server_config_default(pm_server_config* c) {
     c->batch_max_particles = 1 << 17;
     c->device = 0;
     c->pinned_staging_mib = 1024;
-    // the tile-sorted members in the batched launches measured slower on the
-    // 512-request ensemble (4.60-4.64 s against 4.28-4.39 s on the lanes): a
-    // batched round ends when its slowest lattice size's chain does
+    // the tile-sorted members in the batched launches measured as fast as the
+    // lanes on the 512-request ensemble (4.39-4.45 s against 4.34-4.44 s)
     c->batch_tiled = 0;
 }
 

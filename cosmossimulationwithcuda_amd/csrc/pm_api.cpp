@@ -66,9 +66,15 @@ int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
 // FP32: Stockham radix 4/8 (default plan for the size); BF16X3: radix-16
 // matrix-core stages, one row pair / column per block.
-// Re-sort every 7 steps by default: measured 0.927 ms per step on the
-// headline case against 0.931 at 8, 0.930 at 6 and 1.03 at 10 (the re-sort
-// step itself grows with the particles' drift since the last sort).
+// Re-sort every 7 steps at load; the drift rule (cycle_boundary) shortens the
+// interval as the particles speed up.  The counting and re-sorting steps of
+// a cycle grow with the drift since the last sort (particles outside their
+// tile windows, scattered re-sort writes).  Two colliding disks, 1e8
+// particles, 2048^2, CIC, ms per step over steps 5-25 / 25-61 / 61-200:
+// fixed 7 0.876 / 0.861 / 0.985, fixed 5 0.887 / 0.879 / 0.908, 7 with the
+// drift rule (2000 ppm) 0.876 / 0.861 / 0.907; the 2e7 single disk over its
+// 500 steps 0.361 ms at fixed 5, 0.312 with the rule (down to 3); NGP 1e8
+// disks 0.86-0.87 either way.
 constexpr int kResortIntervalDefault = 7;
 // Large dense CIC runs (at least 3e7 particles, more than 64 per cell on
 // average) re-sort every 2 steps with the separate sort: there the fused
@@ -76,8 +82,8 @@ constexpr int kResortIntervalDefault = 7;
 // global atomics each) cost more than a sort.  Measured on two colliding
 // disks of 1e9 particles on 2048^2 over steps 5-30 and 30-55: CIC 16.3 and
 // 22.1 ms per step at the default, 14.4 and 14.0 at 2; NGP 10.0 and 10.9 at
-// the default, 14.3 and 13.8 at 2; 1e8 and 2e7 particles are fastest at the
-// default.  Chosen at each load unless an interval was set.
+// the default, 14.3 and 13.8 at 2.  Chosen at each load unless an interval
+// was set.
 int auto_resort_interval(int mode, int64_t np, int n) {
     const bool dense = np > (int64_t)pmk::kPipelineMaxDensity * n * n;
     return mode == PM_CIC && np >= pmk::kSmallRunParticles && dense ? 2 : kResortIntervalDefault;
@@ -207,6 +213,29 @@ pm_status upload_field_desc(pm_sim* h) {
 }
 
 // ---- stages -----------------------------------------------------------------
+// A re-sort cycle begins (every push so far ran in the old order).  The cycle
+// before the one that just ended has all its migrant reads in (they lag at
+// most two pushes and a cycle is at least three), so its peak rate steers
+// the automatic interval: as the particles speed up (collapsing disks), the
+// step kernels of a long cycle's last steps find more and more of them
+// outside their tile windows (global atomics) and the re-sorting step
+// scatters more of them across tiles.  The interval only shortens (a rule
+// that also lengthened it below a lower rate oscillated and was slower).
+// Intervals never change results.
+void cycle_boundary(pm_sim* h) {
+    if (h->resort_auto && h->resort_permille > 0 && h->drift_ppm > 0 && h->resort_base >= 4) {
+        const double r = h->prev_peak * 1e6 / (double)std::max<int64_t>(h->np, 1);
+        if (r > (double)h->drift_ppm && h->resort_interval > 3) {
+            h->resort_interval -= 1;
+            h->interval_changes += 1;
+        }
+    }
+    h->prev_peak = h->cyc_peak;
+    h->cyc_peak = 0.0;
+    h->prev_begin = h->cyc_begin;
+    h->cyc_begin = h->pushes;
+}
+
 pm_status sort_now(pm_sim* h, bool counted) {
     h->scatter_pending = false;
     h->resort_request = false;
@@ -215,6 +244,7 @@ pm_status sort_now(pm_sim* h, bool counted) {
     std::swap(h->cur, h->alt);
     h->steps_since_sort = 0;
     h->sorts += 1;
+    cycle_boundary(h);
     return PM_OK;
 }
 
@@ -306,6 +336,8 @@ pm_status adaptive_resort(pm_sim* h, bool fused) {
     const double per_step = steps > 0 ? (double)(v - h->mig_last) / (double)steps : 0.0;
     h->mig_last = v;
     h->pushes_at_read = h->pushes_at_copy;
+    if (h->pushes_at_copy > h->cyc_begin) h->cyc_peak = std::max(h->cyc_peak, per_step);
+    else if (h->pushes_at_copy > h->prev_begin) h->prev_peak = std::max(h->prev_peak, per_step);
     if (h->steps_since_sort > 0 && per_step * 1000.0 > (double)h->resort_permille * (double)h->np) {
         h->adaptive_sorts += 1;
         if (fused) {
@@ -354,6 +386,7 @@ pm_status stage_push_deposit(pm_sim* h, bool next_cleared) {
         h->steps_since_sort = 0;
         h->sorts += 1;
         h->pushes += 1;
+        cycle_boundary(h);
         return PM_OK;
     }
     h->scatter_pending = false;
@@ -421,7 +454,12 @@ pm_status upload(pm_sim* h, const float* x, const float* y, const float* vx, con
     }
     h->np = n;
     h->shift = h->cfg.mass_shift > 0 ? h->cfg.mass_shift : auto_shift(m, n);
-    if (h->resort_auto) h->resort_interval = auto_resort_interval(h->cfg.mode, n, h->cfg.n_lattice);
+    if (h->resort_auto) {
+        h->resort_interval = auto_resort_interval(h->cfg.mode, n, h->cfg.n_lattice);
+        h->resort_base = h->resort_interval;
+    }
+    h->cyc_begin = h->prev_begin = h->pushes;
+    h->cyc_peak = h->prev_peak = 0.0;
     const int nl = h->cfg.n_lattice;
     const bool fused_dense = n > (int64_t)pmk::kPipelineMaxDensity * nl * nl && h->fused_sort &&
                              h->resort_interval >= 3 && pmk::scatter_fits(nl, h->bins.tl);
@@ -906,6 +944,11 @@ pm_status pm_set_param(pm_handle h, const char* name, int64_t value) {
         h->resort_permille = (int)value;
         return PM_OK;
     }
+    if (k == "resort_drift_ppm") {
+        if (value < 0 || value > 1000000) return fail(h, PM_EINVAL, "resort_drift_ppm must be in [0, 1e6]");
+        h->drift_ppm = (int)value;
+        return PM_OK;
+    }
     if (k == "deposit_chunk") {
         if (value < pmk::kDepositBlock || value > (1 << 24) || (value % pmk::kDepositBlock) != 0)
             return fail(h, PM_EINVAL, "deposit_chunk must be a multiple of 512 in [512, 2^24]");
@@ -948,6 +991,8 @@ pm_status pm_get_param(pm_handle h, const char* name, int64_t* value) {
     const std::string k(name);
     if (k == "resort_interval") *value = h->resort_interval;
     else if (k == "resort_migrants_permille") *value = h->resort_permille;
+    else if (k == "resort_drift_ppm") *value = h->drift_ppm;
+    else if (k == "interval_changes") *value = h->interval_changes;
     else if (k == "bin_max_chunks") *value = h->bins.max_chunks_per_bin;
     else if (k == "fused_sort") *value = h->fused_sort;
     else if (k == "adaptive_sorts") *value = h->adaptive_sorts;

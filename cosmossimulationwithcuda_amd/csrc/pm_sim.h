@@ -56,6 +56,15 @@ struct pm_sim {
     unsigned long long mig_last = 0;
     int64_t pushes = 0, pushes_at_copy = 0, pushes_at_read = 0;
     int64_t adaptive_sorts = 0;
+    // drift-following interval (automatic interval, adaptive re-sort on): the
+    // peak migrant rate of each re-sort cycle, its reads attributed to cycles
+    // by push index, shortens the interval by one (down to 3) when above
+    // drift_ppm particles per million per step
+    int resort_base = 7;                   // interval chosen at load
+    int drift_ppm = 2000;
+    int64_t cyc_begin = 0, prev_begin = 0;  // pushes done when the current / previous cycle began
+    double cyc_peak = 0.0, prev_peak = 0.0;
+    int64_t interval_changes = 0;
     int64_t sorts = 0;
     unsigned long long* migrants = nullptr;
     int64_t* rho_buf[2] = {nullptr, nullptr};

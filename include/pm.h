@@ -83,7 +83,9 @@ typedef struct pm_config {
     int32_t device;        /* HIP device ordinal                                       */
     int32_t resort_interval; /* steps between re-binning the particles by lattice tile;
                                 0 = chosen at each load: 2 for CIC runs of at least
-                                3e7 particles with more than 64 per cell, else 7.
+                                3e7 particles with more than 64 per cell, else 7,
+                                shortened as the particles drift faster (tunable
+                                "resort_drift_ppm").
                                 Affects speed only, never results.                    */
     int32_t reserved[6];
 } pm_config;
@@ -186,7 +188,11 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
 /* Runtime tunables: "resort_interval"
  * (steps between particle re-binnings), "resort_migrants_permille" (re-bin
  * early when more than this many per thousand particles per step land outside
- * their tile's window; 0 = off, default 10), "deposit_chunk" (particles per
+ * their tile's window; 0 = off, default 10), "resort_drift_ppm" (with the
+ * interval chosen at load and the adaptive re-sort on: shorten the interval
+ * by one, down to 3, after a re-sort cycle whose peak rate of particles
+ * outside their tile windows exceeded this many per million per step;
+ * 0 = off, default 2000), "deposit_chunk" (particles per
  * particle-kernel block, a multiple of 512; by default set at each load:
  * below 3e7 particles about n_particles / 256 within [2048, 8192]; above,
  * 7680 on lattices up to 2048 and 20480 above, or about n_particles / 8192

@@ -567,8 +567,9 @@ def test_deposit_chunk_follows_particle_count():
 
 def test_resort_interval_follows_density():
     """Large dense CIC runs (>= 3e7 particles, > 64 per cell) re-sort every 2
-    steps with the separate sort, others every 7; an interval set explicitly
-    is kept; the schedule never changes results (bit-identical trajectories)."""
+    steps with the separate sort, others every 7 at load; an interval set
+    explicitly is kept; the schedule never changes results (bit-identical
+    trajectories)."""
     n, npart = 512, 30_000_000
     x, y, vx, vy, m = ics.uniform_square(npart, 32, 480, seed=9)
     outs = []
@@ -583,6 +584,7 @@ def test_resort_interval_follows_density():
                 outs.append(s.read())
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
 
 
 def test_autotune_keeps_state():
@@ -629,6 +631,33 @@ def test_adaptive_resort_keeps_results():
     (a, sorts_off), (b, sorts_on) = out
     print("adaptive sorts:", sorts_off, sorts_on)
     assert sorts_off == 0 and sorts_on > 0
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v, equal_nan=True)
+
+
+def test_drift_rule_shortens_interval_keeps_results():
+    """With the interval chosen at load, cycles whose particles drift out of
+    their tile windows shorten it (down to 3); with the rule off it stays at
+    7.  Positions and velocities are bit-identical either way.  2^20
+    particles: the migrant reads then come at a fixed lag of two steps."""
+    n, k = 512, 1 << 20
+    x, y, _, _, m = ics.uniform_square(k, 128, 384, seed=23)
+    rng = np.random.default_rng(24)
+    vx = rng.normal(0.0, 150.0, k).astype(np.float32)      # ~1.5 cells per step at dt 0.01
+    vy = rng.normal(0.0, 150.0, k).astype(np.float32)
+    out = []
+    for ppm in (0, 2000):
+        with make_sim(n, k, pm.CIC) as s:
+            s.set_param("resort_drift_ppm", ppm)
+            assert s.get_param("resort_drift_ppm") == ppm
+            s.load(x, y, vx, vy, m)
+            s.step(40)
+            s.sync()
+            out.append((s.read(), s.get_param("resort_interval"), s.get_param("interval_changes")))
+    (a, iv_off, ch_off), (b, iv_on, ch_on) = out
+    print("interval off/on:", iv_off, iv_on, "changes", ch_off, ch_on)
+    assert iv_off == 7 and ch_off == 0
+    assert 3 <= iv_on < 7 and ch_on == 7 - iv_on
     for u, v in zip(a, b):
         assert np.array_equal(u, v, equal_nan=True)
 

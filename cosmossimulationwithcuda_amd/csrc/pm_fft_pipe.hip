@@ -6,8 +6,8 @@
 // transform as the one-per-block passes of pm_fft.hip (identical results);
 // the C2R pass stages the potential row pair and both force rows of a pair in
 // one batch (three transforms), which frees the prefetch registers before the
-// transform starts.  With the FP32 stages only the C2R pass runs IT per block
-// (launch_pass).  Plan field `it` (rp = cb = 1); compiled for the default
+// transform starts.  Only the C2R pass runs IT per block (launch_pass).
+// Plan field `it` (rp = cb = 1); compiled for the default
 // radix at N = 1024 (radix 4), 2048 and 4096 (radix 8), IT in {2, 4}, and for
 // the split-BF16 matrix-core radix codes 16 and 17 at N = 1024..4096, IT = 2.
 #include "pm_fft_pipe_core.h"
@@ -60,17 +60,18 @@ hipError_t launch_c2r(const FieldDesc* d, int count, const float2* tw, bool forc
     return hipGetLastError();
 }
 
-// pass 0 = R2C, 1 = column, 2 = C2R.  With the FP32 stages the R2C and
-// column passes run one row pair / column per block (twice the blocks in
-// flight; measured faster than two in sequence with the prefetch, alone and
-// batched: 2048^2 R2C 19.5 -> 18.6 us, column 27.0 -> 24.8 us, per instance
-// of 16 batched 17.2 -> 14.8 and 18.3 -> 13.8 us; 4096^2 92 -> 83 and 91 ->
-// 74 us); the C2R pass keeps IT row pairs per block, which its edge bands
-// need (the last pair reads the potential rows of the pair before it).
+// pass 0 = R2C, 1 = column, 2 = C2R.  The R2C and column passes run one row
+// pair / column per block (twice the blocks in flight; measured faster than
+// two in sequence with the prefetch, alone and batched.  FP32: 2048^2 R2C
+// 19.5 -> 18.6 us, column 27.0 -> 24.8 us, per instance of 16 batched 17.2
+// -> 14.8 and 18.3 -> 13.8 us; 4096^2 92 -> 83 and 91 -> 74 us.  Split-BF16
+// at 4096^2: three-term column 255 -> 183 us, R2C 112 -> 93 us); the C2R
+// pass keeps IT row pairs per block, which its edge bands need (the last
+// pair reads the potential rows of the pair before it).
 template <int N, int RADIX, int IT>
 hipError_t launch_pass(int pass, const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
     static_assert(padded(3 * N) * 8 + 8 * tw_lds_entries(N, RADIX) <= 160 * 1024, "C2R batch must fit LDS");
-    constexpr int IT_RC = RADIX < 16 ? 1 : IT;
+    constexpr int IT_RC = 1;
     if (pass == 0) return launch_r2c<N, RADIX, IT_RC>(d, count, tw, s);
     if (pass == 1) return launch_col<N, RADIX, IT_RC>(d, count, tw, force, s);
     return launch_c2r<N, RADIX, IT>(d, count, tw, force, s);

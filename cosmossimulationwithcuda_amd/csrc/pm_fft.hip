@@ -404,11 +404,13 @@ bool plan_valid(int n, FftPlan p) {
 
 // Measured on MI355X over n = 256..4096: one row pair / one column per block
 // (the most blocks in flight) beats larger batches at every size; radix 8 from
-// 2048 up.  Two row pairs / columns per block in sequence, with the next one
-// prefetched and the edge bands done in the C2R pass, take the field solve
-// from 64.7 to 58.6 us at 1024, 107 to 97 us at 2048 and 0.36 to 0.34 ms at
-// 4096 (with 512-thread blocks there); four per block are slower.  At 256 and
-// 512 they serve batches of simulations (one edge launch fewer per step).
+// 2048 up.  The C2R pass with two row pairs per block in sequence, the next
+// one prefetched and the edge bands done in the pass (the R2C and column
+// passes one per block), takes the field solve from 64.7 to 58.6 us at 1024,
+// 107 to 97 us at 2048 and 0.36 to 0.34 ms at 4096 (with 512-thread blocks
+// there), measured with two per block in all three passes, which the R2C and
+// column passes have since left; four per block are slower.  At 256 and 512
+// it serves batches of simulations (one edge launch fewer per step).
 FftPlan default_plan(int n) {
     FftPlan p;
     p.radix = n >= 2048 ? 8 : 4;

@@ -204,10 +204,12 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * 2^6 cells; not below the lattice's default) change speed, never results;
  * the FFT plan "fft_rp", "fft_cb", "fft_radix" changes FP32 rounding only
  * (see pm_autotune; "fft_radix" 64 selects the wave-local passes, one
- * 64-lane wavefront per row pair / column, N = 256 .. 2048); "fft_it" (row pairs / columns per FFT-pass block,
- * handled in sequence with the next one's inputs prefetched: 1, or 2 or 4 with
- * fft_rp = fft_cb = 1 at N = 256, 512 (2 only) and 1024 radix 4 and N = 2048,
- * 4096 radix 8; default
+ * 64-lane wavefront per row pair / column, N = 256 .. 2048); "fft_it" (row
+ * pairs per block of the C2R pass, handled in sequence with the next one's
+ * inputs prefetched and the edge bands done in the pass, the R2C and column
+ * passes then running one row pair / column per block: 1 (the one-per-block
+ * passes and a separate edge pass), or 2 or 4 with fft_rp = fft_cb = 1 at
+ * N = 256, 512 (2 only) and 1024 radix 4 and N = 2048, 4096 radix 8; default
  * 2 at those sizes, else 1; setting another batch or radix resets it to 1
  * where needed) changes speed only.  pm_get_param also reads "adaptive_sorts" (early
  * re-bins so far).  No counterpart in the reference, whose

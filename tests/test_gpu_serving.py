@@ -13,11 +13,13 @@ from cosmossimulationwithcuda_amd import ics, pm
 pytestmark = pytest.mark.gpu
 
 
-def request(n, npart, mode, seed, **kw):
+def request(n, npart, mode, seed, shift=0.0, **kw):
+    """shift: the disk's centre moves by this many cells in x (particles
+    outside [1, N-1) stand parked: no deposit, no motion)."""
     cfg = dict(n_lattice=n, n_particles=npart, mode=mode, dt=0.02, gravity=1.0, green_origin=1.0,
                deconv_eps=1e-3)
     cfg.update(kw)
-    x, y, vx, vy, m = ics.exponential_disk(npart, n / 2, n / 2, n / 12, n / 3, seed=seed)
+    x, y, vx, vy, m = ics.exponential_disk(npart, n / 2 + shift, n / 2, n / 12, n / 3, seed=seed)
     return pm.default_config(**cfg), (x, y, vx, vy, m)
 
 
@@ -43,6 +45,8 @@ MIX = [  # (lattice, particles, mode, steps, extra config)
     (1024, 150000, pm.NGP, 16, {}),
     (512, 30000, pm.CIC, 21, {"force_mode": pm.FORCE_STENCIL}),
     (512, 40000, pm.NGP, 29, {"wrap_mode": pm.WRAP_REFERENCE}),
+    (512, 15000, pm.CIC, 19, {"shift": 700.0}),      # every particle parked: a member without chunks
+    (1024, 30000, pm.NGP, 23, {"shift": 400.0}),     # the disk's outer part parked
 ]
 
 

@@ -212,7 +212,8 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * N = 256, 512 (2 only) and 1024 radix 4 and N = 2048, 4096 radix 8; default
  * 2 at those sizes, else 1; setting another batch or radix resets it to 1
  * where needed) changes speed only.  pm_get_param also reads "adaptive_sorts" (early
- * re-bins so far).  No counterpart in the reference, whose
+ * re-bins so far) and "interval_changes" (shortenings of the interval by the
+ * drift rule so far).  No counterpart in the reference, whose
  * launch shape is the compile-time Constants::THREADS (cuh:31). */
 pm_status pm_set_param(pm_handle h, const char* name, int64_t value);
 pm_status pm_get_param(pm_handle h, const char* name, int64_t* value);

@@ -8,10 +8,11 @@ At these sizes the checks are the ones a CPU run of seconds can back:
     lattice, then the deposit of the new positions);
   * the field of that step against the FP64 oracle (relative L2 <= 1e-5,
     headline case);
-  * invariants over further steps that cross fused re-sort cycles: the
-    particle mass sum and in-bounds count unchanged while every particle stays
-    inside the lattice, a finite state, and momentum drift small against the
-    momentum scale (1e-3 of sum m |v|).
+  * invariants over the rest of the configuration's 200 steps, across the
+    re-sort cycles and the interval changes of the drift rule: the particle
+    mass sum and in-bounds count unchanged while every particle stays inside
+    the lattice, a finite state, and momentum drift small against the
+    momentum scale (1e-6 of sum m |v|; measured below 1e-8).
 """
 import time
 
@@ -62,7 +63,7 @@ def check_invariants(d0, d, npart, arrs):
     x, y, vx, vy, m = arrs
     pscale = float(np.sum(m.astype(np.float64) * np.hypot(vx, vy)))
     assert d["mass"] == d0["mass"] and d["in_bounds"] == npart and d["nonfinite"] == 0
-    assert abs(d["px"] - d0["px"]) < 1e-3 * pscale and abs(d["py"] - d0["py"]) < 1e-3 * pscale
+    assert abs(d["px"] - d0["px"]) < 1e-6 * pscale and abs(d["py"] - d0["py"]) < 1e-6 * pscale
 
 
 def test_headline_full_size():
@@ -83,11 +84,11 @@ def test_headline_full_size():
                             / np.sum(fx64 ** 2 + fy64 ** 2)))
         say(f"field vs FP64: potential {e_phi:.2e}, force {e_f:.2e}", t0)
         assert e_phi <= REL_L2_FIELD and e_f <= REL_L2_FIELD
-        s.step(30)                                 # crosses fused re-sort cycles (interval 7)
+        s.step(199)                                # the configuration's 200 steps
         d = s.diagnostics()
-        assert s.info().sorts >= 3
+        assert s.info().sorts >= 20
     check_invariants(d0, d, npart, arrs)
-    say("invariants after 31 steps", t0)
+    say("invariants after 200 steps", t0)
 
 
 def test_one_gpu_scale_ngp():
@@ -101,10 +102,10 @@ def test_one_gpu_scale_ngp():
         s.load(*arrs)
         d0 = s.diagnostics()
         check_fused_step(s, n, pm.NGP, dt, arrs, t0)
-        s.step(12)                                 # adaptive re-sorts fire every few steps at this mass
+        s.step(199)                                # the configuration's 200 steps
         d = s.diagnostics()
     check_invariants(d0, d, npart, arrs)
-    say("invariants after 13 steps", t0)
+    say("invariants after 200 steps", t0)
 
 
 def test_one_gpu_scale_cic():
@@ -122,8 +123,8 @@ def test_one_gpu_scale_cic():
         d0 = s.diagnostics()
         sorts0 = s.info().sorts
         check_fused_step(s, n, pm.CIC, dt, arrs, t0)
-        s.step(12)
+        s.step(199)                                # the configuration's 200 steps
         d = s.diagnostics()
-        assert s.info().sorts - sorts0 >= 6               # the sort cadence was crossed
+        assert s.info().sorts - sorts0 >= 90              # a sort every 2 steps
     check_invariants(d0, d, npart, arrs)
-    say("invariants after 13 steps", t0)
+    say("invariants after 200 steps", t0)

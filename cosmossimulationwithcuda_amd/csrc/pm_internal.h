@@ -11,8 +11,19 @@ constexpr int kBlock = 256;   // threads per block for every streaming kernel (4
 // Particle binning: square tiles of 2^tl cells; a particle kernel block works
 // on one chunk of one tile with LDS windows covering the tile plus a
 // kHalo-cell margin (plus the +1 column/row CIC reaches).  Tiles are 32 cells
-// for lattices up to 2048 (at most 4096 bins, 17.7 KB windows) and 64 above.
-constexpr int kHalo = 7;
+// for lattices up to 2048 (at most 4096 bins, 24.2 KB per window) and 64
+// above.  The margin is the largest with which the re-sorting step (two
+// windows and two per-bin tables) still runs two blocks per CU at 32-cell
+// tiles: fewer particles leave their windows between re-sorts (each such
+// deposit is a global atomic), so the drift rule keeps longer intervals.
+// Against 7 cells, ms per step: colliding disks 1e8 CIC over steps
+// 5-25 / 25-61 / 61-200 0.868-0.883 / 0.850 / 0.863 against 0.870 / 0.854 /
+// 0.903; single disk 2e7 over steps 325-500 0.290 against 0.328; 1e9 NGP
+// over steps 15-35 9.36 against 9.92; 1e9 CIC over steps 35-55 13.10
+// against 14.33; 12 cells: 0.986 / 0.947 / 0.948 (one re-sorting block per
+// CU).  At 64-cell tiles (4096^2 Plummer binary, 20 steps) 1.204 against
+// 1.193 ms.
+constexpr int kHalo = 11;
 constexpr int kDepositBlock = 512;
 constexpr int kDepositChunkDefault = 12288;          // particles per particle-kernel block (tunable)
 // Automatic chunk, chosen at each load:

@@ -116,7 +116,7 @@ template <int TL> constexpr int fused_block() { return TL <= 5 ? 512 : 1024; }
 // particles (a sparse tile of a small or spread-out simulation) gathers from
 // the force lattice and deposits with global atomics instead of staging the
 // two tile windows in LDS: the same arithmetic, bit for bit, without loading
-// and flushing the 35 KB of windows (32-cell tiles) for a few dozen particles.
+// and flushing the 48 KB of windows (32-cell tiles) for a few dozen particles.
 constexpr int kSparseChunk = 256;
 template <int TL, int MODE, int WRAP, bool COUNT, bool PIPE>
 __device__ __forceinline__ void push_deposit_block(

@@ -644,8 +644,8 @@ def test_drift_rule_shortens_interval_keeps_results():
     n, k = 512, 1 << 20
     x, y, _, _, m = ics.uniform_square(k, 128, 384, seed=23)
     rng = np.random.default_rng(24)
-    vx = rng.normal(0.0, 150.0, k).astype(np.float32)      # ~1.5 cells per step at dt 0.01
-    vy = rng.normal(0.0, 150.0, k).astype(np.float32)
+    vx = rng.normal(0.0, 300.0, k).astype(np.float32)      # ~3 cells per step at dt 0.01
+    vy = rng.normal(0.0, 300.0, k).astype(np.float32)
     out = []
     for ppm in (0, 2000):
         with make_sim(n, k, pm.CIC) as s:

@@ -66,16 +66,21 @@ int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
 // FP32: Stockham radix 4/8 (default plan for the size); BF16X3: radix-16
 // matrix-core stages, one row pair / column per block.
-// Re-sort every 7 steps at load; the drift rule (cycle_boundary) shortens the
-// interval as the particles speed up.  The counting and re-sorting steps of
-// a cycle grow with the drift since the last sort (particles outside their
-// tile windows, scattered re-sort writes).  Two colliding disks, 1e8
-// particles, 2048^2, CIC, ms per step over steps 5-25 / 25-61 / 61-200:
-// fixed 7 0.876 / 0.861 / 0.985, fixed 5 0.887 / 0.879 / 0.908, 7 with the
-// drift rule (2000 ppm) 0.876 / 0.861 / 0.907; the 2e7 single disk over its
-// 500 steps 0.361 ms at fixed 5, 0.312 with the rule (down to 3); NGP 1e8
-// disks 0.86-0.87 either way.
-constexpr int kResortIntervalDefault = 7;
+// Re-sort every 9 steps at load (7 for dense runs, more than 64 particles per
+// cell); the drift rule (cycle_boundary) shortens the interval as the
+// particles speed up.  The counting and re-sorting steps of a cycle grow with
+// the drift since the last sort (particles outside their tile windows,
+// scattered re-sort writes).  Two colliding disks, 1e8 particles, 2048^2,
+// CIC, 11-cell window margin, ms per step over steps 5-25 / 25-61 / 61-200:
+// 7 with the rule at 2000 per million 0.871-0.878 / 0.849-0.855 / 0.862; 9 at
+// 2000 0.852-0.856 / 0.838 / 0.893 (the rule does not fire: few particles
+// leave the wider windows, the re-sort steps grow instead); 9 at 200 0.851-
+// 0.854 / 0.840-0.843 / 0.862-0.865; fixed 8, 9, 10 late 0.879 / 0.893 /
+// 0.928.  NGP disks 1e8 over steps 5-25: 0.830 at 9 against 0.844 at 7; the
+// 2e7 single disk the same either way; 1e9 NGP (dense) over steps 5-15 11.40
+// at 9 against 9.93 at 7.
+constexpr int kResortIntervalDefault = 9;
+constexpr int kResortIntervalDense = 7;
 // Large dense CIC runs (at least 3e7 particles, more than 64 per cell on
 // average) re-sort every 2 steps with the separate sort: there the fused
 // re-sort step and the window spills of drifted particles (four contended
@@ -86,7 +91,8 @@ constexpr int kResortIntervalDefault = 7;
 // was set.
 int auto_resort_interval(int mode, int64_t np, int n) {
     const bool dense = np > (int64_t)pmk::kPipelineMaxDensity * n * n;
-    return mode == PM_CIC && np >= pmk::kSmallRunParticles && dense ? 2 : kResortIntervalDefault;
+    if (!dense) return kResortIntervalDefault;
+    return mode == PM_CIC && np >= pmk::kSmallRunParticles ? 2 : kResortIntervalDense;
 }
 
 pmk::FftPlan plan_for(const pm_config& c) {

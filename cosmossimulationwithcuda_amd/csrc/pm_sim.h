@@ -40,7 +40,7 @@ struct pm_sim {
     int nbins = 0;                         // in-bounds bins + parked bin
     int nbins_alloc = 0;                   // bins the bin arrays were sized for
     int max_chunks = 0;
-    int resort_interval = 7;
+    int resort_interval = 9;
     bool resort_auto = true;               // interval chosen at load by auto_resort_interval
     int steps_since_sort = 0;
     // adaptive re-sort: the migrant counter is copied to pinned host memory
@@ -60,8 +60,8 @@ struct pm_sim {
     // peak migrant rate of each re-sort cycle, its reads attributed to cycles
     // by push index, shortens the interval by one (down to 3) when above
     // drift_ppm particles per million per step
-    int resort_base = 7;                   // interval chosen at load
-    int drift_ppm = 2000;
+    int resort_base = 9;                   // interval chosen at load
+    int drift_ppm = 200;
     int64_t cyc_begin = 0, prev_begin = 0;  // pushes done when the current / previous cycle began
     double cyc_peak = 0.0, prev_peak = 0.0;
     int64_t interval_changes = 0;

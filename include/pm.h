@@ -82,9 +82,10 @@ typedef struct pm_config {
     int32_t mass_shift;    /* fixed-point bits (rho = q * 2^-shift); 0 = from the masses */
     int32_t device;        /* HIP device ordinal                                       */
     int32_t resort_interval; /* steps between re-binning the particles by lattice tile;
-                                0 = chosen at each load: 2 for CIC runs of at least
-                                3e7 particles with more than 64 per cell, else 7,
-                                shortened as the particles drift faster (tunable
+                                0 = chosen at each load: 9, or 7 with more than 64
+                                particles per cell, or 2 for CIC runs of at least
+                                3e7 particles with more than 64 per cell; shortened
+                                as the particles drift faster (tunable
                                 "resort_drift_ppm").
                                 Affects speed only, never results.                    */
     int32_t reserved[6];
@@ -192,7 +193,7 @@ pm_status pm_checkpoint_open(const char* path, int32_t device, pm_handle* out);
  * interval chosen at load and the adaptive re-sort on: shorten the interval
  * by one, down to 3, after a re-sort cycle whose peak rate of particles
  * outside their tile windows exceeded this many per million per step;
- * 0 = off, default 2000), "deposit_chunk" (particles per
+ * 0 = off, default 200), "deposit_chunk" (particles per
  * particle-kernel block, a multiple of 512; by default set at each load:
  * below 3e7 particles about n_particles / 256 within [2048, 8192]; above,
  * 7680 on lattices up to 2048 and 20480 above, or about n_particles / 8192

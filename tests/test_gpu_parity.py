@@ -568,9 +568,9 @@ def test_deposit_chunk_follows_particle_count():
 
 def test_resort_interval_follows_density():
     """Large dense CIC runs (>= 3e7 particles, > 64 per cell) re-sort every 2
-    steps with the separate sort, others every 7 at load; an interval set
-    explicitly is kept; the schedule never changes results (bit-identical
-    trajectories)."""
+    steps with the separate sort, other dense runs every 7 and sparser ones
+    every 9 at load; an interval set explicitly is kept; the schedule never
+    changes results (bit-identical trajectories)."""
     n, npart = 512, 30_000_000
     x, y, vx, vy, m = ics.uniform_square(npart, 32, 480, seed=9)
     outs = []
@@ -585,6 +585,11 @@ def test_resort_interval_follows_density():
                 outs.append(s.read())
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    x, y, vx, vy, m = ics.uniform_square(1_000_000, 32, 480, seed=9)      # ~4 per cell
+    for mode in (pm.CIC, pm.NGP):
+        with make_sim(n, 1_000_000, mode, dt=0.05) as s:
+            s.load(x, y, vx, vy, m)
+            assert s.get_param("resort_interval") == 9
 
 
 
@@ -637,10 +642,11 @@ def test_adaptive_resort_keeps_results():
 
 
 def test_drift_rule_shortens_interval_keeps_results():
-    """With the interval chosen at load, cycles whose particles drift out of
-    their tile windows shorten it (down to 3); with the rule off it stays at
-    7.  Positions and velocities are bit-identical either way.  2^20
-    particles: the migrant reads then come at a fixed lag of two steps."""
+    """With the interval chosen at load (9 at this density), cycles whose
+    particles drift out of their tile windows shorten it (down to 3); with the
+    rule off it stays at 9.  Positions and velocities are bit-identical either
+    way.  2^20 particles: the migrant reads then come at a fixed lag of two
+    steps."""
     n, k = 512, 1 << 20
     x, y, _, _, m = ics.uniform_square(k, 128, 384, seed=23)
     rng = np.random.default_rng(24)
@@ -657,8 +663,8 @@ def test_drift_rule_shortens_interval_keeps_results():
             out.append((s.read(), s.get_param("resort_interval"), s.get_param("interval_changes")))
     (a, iv_off, ch_off), (b, iv_on, ch_on) = out
     print("interval off/on:", iv_off, iv_on, "changes", ch_off, ch_on)
-    assert iv_off == 7 and ch_off == 0
-    assert 3 <= iv_on < 7 and ch_on == 7 - iv_on
+    assert iv_off == 9 and ch_off == 0
+    assert 3 <= iv_on < 9 and ch_on == 9 - iv_on
     for u, v in zip(a, b):
         assert np.array_equal(u, v, equal_nan=True)
 

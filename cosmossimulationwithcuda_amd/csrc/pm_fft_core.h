@@ -180,8 +180,9 @@ __device__ __forceinline__ int row_group() {
 
 // Threads per block of the FFT passes: 512 for N >= 4096 with the FP32
 // stages (twice the waves per CU at the same LDS footprint: 4096^2 field
-// solve 0.36-0.41 -> 0.34 ms), else 256 (512 is slower at 2048 with two row
-// pairs per block; the matrix-core radix-16 stages are written for 256).
+// solve 0.36-0.41 -> 0.34 ms), else 256 (the R2C and column passes are no
+// faster with 512 at 2048; the matrix-core radix-16 stages are written for
+// 256).  The C2R pass picks its own (c2r_threads, pm_fft_pipe.hip).
 template <int N, int RADIX>
 constexpr int fft_threads() { return N >= 4096 && RADIX < 16 ? 512 : kBlock; }
 

@@ -31,13 +31,28 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv_it(const
     col_pipe_body<N, RADIX, FORCE, IT>(gfield(descs[blockIdx.y]), tw, buf, twl, blockIdx.x * IT);
 }
 
+// Threads of the C2R pass's blocks.  With the force rows a block holds three
+// transforms in LDS (57 KB at 2048^2), so at most two blocks share a CU, and
+// at 256 threads the prefetch registers (206 VGPRs) allow two waves per SIMD.
+// 512 threads with the registers capped at 128 (no spills) keep four waves
+// per SIMD: 2048^2 C2R 38.8-39.4 -> 34.4-34.7 us, 1024^2 (256 blocks, one
+// per CU) 17.9-18.2 -> 16.2-16.3 us (HIP events, field report).  At 4096^2
+// (114 KB, one block per CU) 1024 threads would need 42 spilled registers.
+template <int N, int RADIX>
+constexpr int c2r_threads() {
+    return RADIX < 16 && (N == 1024 || N == 2048) ? 512 : fft_threads<N, RADIX>();
+}
+template <int N, int RADIX, bool FORCE>
+constexpr int c2r_min_waves() { return FORCE && RADIX < 16 && N == 2048 ? 4 : 1; }
+
 template <int N, int RADIX, bool FORCE, int IT>
-__global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_row_c2r_it(const FieldDesc* __restrict__ descs,
-                                                        const float2* __restrict__ tw) {
+__global__ __launch_bounds__((c2r_threads<N, RADIX>()), (c2r_min_waves<N, RADIX, FORCE>())) void k_row_c2r_it(
+    const FieldDesc* __restrict__ descs, const float2* __restrict__ tw) {
     constexpr int ROWS = FORCE ? 3 : 1;
     __shared__ float2 buf[padded(ROWS * N)];
     __shared__ float2 twl[tw_lds_entries(N, RADIX)];
-    c2r_pipe_body<N, RADIX, FORCE, IT>(gfield(descs[blockIdx.y]), tw, buf, twl, row_group() * IT);
+    c2r_pipe_body<N, RADIX, FORCE, IT, c2r_threads<N, RADIX>()>(gfield(descs[blockIdx.y]), tw, buf, twl,
+                                                                row_group() * IT);
 }
 
 template <int N, int RADIX, int IT>
@@ -55,8 +70,8 @@ hipError_t launch_col(const FieldDesc* d, int count, const float2* tw, bool forc
 template <int N, int RADIX, int IT>
 hipError_t launch_c2r(const FieldDesc* d, int count, const float2* tw, bool force, hipStream_t s) {
     const dim3 grid(N / 2 / IT, count);
-    if (force) hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, true, IT>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
-    else hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, false, IT>), grid, dim3(fft_threads<N, RADIX>()), 0, s, d, tw);
+    if (force) hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, true, IT>), grid, dim3(c2r_threads<N, RADIX>()), 0, s, d, tw);
+    else hipLaunchKernelGGL((k_row_c2r_it<N, RADIX, false, IT>), grid, dim3(c2r_threads<N, RADIX>()), 0, s, d, tw);
     return hipGetLastError();
 }
 

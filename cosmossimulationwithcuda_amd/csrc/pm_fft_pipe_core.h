@@ -151,10 +151,11 @@ __device__ __forceinline__ void col_pipe_body(const GField& d, const float2* __r
 // one batch of 1 or 3 transforms.
 // Row pairs g0 .. g0 + IT - 1 (the edge bands need IT >= 2 with g0 a
 // multiple of IT); buf: padded((FORCE ? 3 : 1) N) float2 of LDS.
-template <int N, int RADIX, bool FORCE, int IT>
+// BT: threads of the block (the pass kernel runs c2r_threads, the round
+// kernel fft_threads).
+template <int N, int RADIX, bool FORCE, int IT, int BT = fft_threads<N, RADIX>()>
 __device__ __forceinline__ void c2r_pipe_body(const GField& d, const float2* __restrict__ tw, float2* buf,
                                               float2* twl, int g0) {
-    constexpr int BT = fft_threads<N, RADIX>();
     constexpr int ROWS = FORCE ? 3 : 1, NK = N / 2 + 1, PK = (NK + BT - 1) / BT;
     const int t = threadIdx.x;
     cplx2 hv[PK], hw[PK];
@@ -200,7 +201,7 @@ __device__ __forceinline__ void c2r_pipe_body(const GField& d, const float2* __r
         }
         lds_barrier();
         if (it + 1 < IT) fetch(g0 + it + 1);    // in flight during the transforms
-        lds_fft<N, ROWS, RADIX, true>(buf, twl);
+        lds_fft<N, ROWS, RADIX, true, BT>(buf, twl);
         for (int i = t; i < N; i += BT) {
             const size_t ra = (size_t)j0 * N + i;
             const float2 z = buf[pad(i)];

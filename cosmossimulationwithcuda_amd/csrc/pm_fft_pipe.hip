@@ -35,12 +35,15 @@ __global__ __launch_bounds__((fft_threads<N, RADIX>())) void k_col_conv_it(const
 // transforms in LDS (57 KB at 2048^2), so at most two blocks share a CU, and
 // at 256 threads the prefetch registers (206 VGPRs) allow two waves per SIMD.
 // 512 threads with the registers capped at 128 (no spills) keep four waves
-// per SIMD: 2048^2 C2R 38.8-39.4 -> 34.4-34.7 us, 1024^2 (256 blocks, one
-// per CU) 17.9-18.2 -> 16.2-16.3 us (HIP events, field report).  At 4096^2
-// (114 KB, one block per CU) 1024 threads spill 42 registers: 140 -> 186 us.
+// per SIMD: 2048^2 C2R 38.8-39.4 -> 34.4-34.7 us; at 1024^2 and 512^2 (256
+// and 128 blocks, at most one per CU) twice the waves per block: 17.9-18.2
+// -> 16.2-16.3 us and 12.1 -> 10.0 us (HIP events, field report).  256^2 (64
+// blocks) stays at 256 threads: N must be a multiple of the block size.  At
+// 4096^2 (114 KB, one block per CU) 1024 threads spill 42 registers: 140 ->
+// 186 us.
 template <int N, int RADIX>
 constexpr int c2r_threads() {
-    return RADIX < 16 && (N == 1024 || N == 2048) ? 512 : fft_threads<N, RADIX>();
+    return RADIX < 16 && N >= 512 && N <= 2048 ? 512 : fft_threads<N, RADIX>();
 }
 template <int N, int RADIX, bool FORCE>
 constexpr int c2r_min_waves() { return FORCE && RADIX < 16 && N == 2048 ? 4 : 1; }

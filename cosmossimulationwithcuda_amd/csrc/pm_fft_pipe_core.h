@@ -156,6 +156,7 @@ __device__ __forceinline__ void col_pipe_body(const GField& d, const float2* __r
 template <int N, int RADIX, bool FORCE, int IT, int BT = fft_threads<N, RADIX>()>
 __device__ __forceinline__ void c2r_pipe_body(const GField& d, const float2* __restrict__ tw, float2* buf,
                                               float2* twl, int g0) {
+    static_assert(N % BT == 0, "the edge bands read back potential stores made by the same thread");
     constexpr int ROWS = FORCE ? 3 : 1, NK = N / 2 + 1, PK = (NK + BT - 1) / BT;
     const int t = threadIdx.x;
     cplx2 hv[PK], hw[PK];
